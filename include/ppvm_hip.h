@@ -1,0 +1,207 @@
+/*
+ * ppvm_hip.h — C ABI of the MI355X (gfx950) Pauli-propagation engine.
+ *
+ * This is the drop-in boundary for the `PauliSum` Heisenberg-propagation path of
+ * QuEraComputing/ppvm.  Every entry point below replaces one method of the
+ * reference's `PauliSum` surface (Rust traits, PyO3 binding); the citation in
+ * each comment names the reference interface it stands in for.  All pointers are
+ * host memory: terms are copied in and out; device arenas are owned by the handle.
+ *
+ * Term layout across the boundary: a term is W = ceil(n_qubits/64) u64 words of
+ * x bits followed (in a separate array) by W u64 words of z bits.  Qubit i is bit
+ * (i mod 64) of word i/64, the little-endian Lsb0 layout of the reference's
+ * `PauliWord<[u8;N]>` planes (crates/ppvm-pauli-word/src/word/data.rs:43-68).
+ * The Pauli on a qubit is (x,z): I=(0,0) X=(1,0) Z=(0,1) Y=(1,1)
+ * (crates/ppvm-pauli-word/src/word/data.rs:203-213).
+ *
+ * Semantics follow the reference exactly:
+ *  - gates never truncate by themselves; `*_many` style calls (a target array)
+ *    are a sequential loop of single gates with no truncation in between
+ *    (crates/ppvm-traits/src/traits/clifford.rs:175-218,
+ *     crates/ppvm-traits/src/traits/branch/rot1.rs:25-45);
+ *  - `ppvm_truncate` drops |c| < min_abs_coeff, then weight > max_pauli_weight
+ *    (crates/ppvm-pauli-sum/src/strategy.rs:19-30,86-89,133), never dropping a
+ *    preserved string (crates/ppvm-pauli-sum/src/sum/data.rs:275-311).
+ *
+ * Errors: every call returns PPVM_OK (0) or an error code; `ppvm_last_error()`
+ * gives a thread-local message.  Where the reference panics (qubit index out of
+ * range, word/data.rs:203-213) or raises ValueError (odd pair list,
+ * crates/ppvm-python-native/src/lib.rs:12-22) this ABI returns PPVM_EINVAL.
+ *
+ * Threading: a handle is used by one host thread at a time (the reference takes
+ * `&mut self` on every gate); different handles may be driven concurrently.
+ * Work is queued on the handle's own HIP stream and may run asynchronously;
+ * calls that return data (len, overlap, export) synchronise.
+ */
+#ifndef PPVM_HIP_H
+#define PPVM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+enum {
+  PPVM_OK = 0,
+  PPVM_EINVAL = 1,  /* bad argument (index out of range, odd pairs, size)     */
+  PPVM_ENOMEM = 2,  /* device or host allocation failed                      */
+  PPVM_EHIP = 3,    /* HIP runtime error                                     */
+  PPVM_ESTATE = 4,  /* handle in an unusable state (e.g. after a HIP error)  */
+  PPVM_ENODEV = 5   /* no usable gfx950 device                               */
+};
+
+/* ---- op codes (for ppvm_apply and the batched programs) ----------------- */
+enum ppvm_opcode {
+  PPVM_OP_NOP = 0,
+  /* Clifford conjugation, crates/ppvm-pauli-sum/src/sum/clifford.rs:37-217 */
+  PPVM_OP_X = 1, PPVM_OP_Y = 2, PPVM_OP_Z = 3, PPVM_OP_H = 4, PPVM_OP_S = 5,
+  PPVM_OP_S_DAG = 6, PPVM_OP_SQRT_X = 7, PPVM_OP_SQRT_X_DAG = 8,
+  PPVM_OP_SQRT_Y = 9, PPVM_OP_SQRT_Y_DAG = 10,
+  PPVM_OP_CNOT = 11, PPVM_OP_CZ = 12, PPVM_OP_CY = 13,
+  /* Pauli rotations exp(-i theta/2 G), crates/ppvm-pauli-sum/src/sum/rot1.rs:55-128,
+     crates/ppvm-pauli-sum/src/sum/rot2.rs:63-184 */
+  PPVM_OP_RX = 20, PPVM_OP_RY = 21, PPVM_OP_RZ = 22,
+  PPVM_OP_RXX = 23, PPVM_OP_RYY = 24, PPVM_OP_RZZ = 25,
+  /* diagonal noise channels, crates/ppvm-pauli-sum/src/sum/noise.rs:17-131 */
+  PPVM_OP_PAULI_ERROR = 40, PPVM_OP_DEPOLARIZE1 = 41, PPVM_OP_DEPOLARIZE2 = 42,
+  PPVM_OP_PAULI_ERROR2 = 43,
+  /* explicit truncation, crates/ppvm-pauli-sum/src/sum/data.rs:275 */
+  PPVM_OP_TRUNCATE = 60
+};
+
+/* One operation of a program.  q1 is the second qubit of two-qubit ops;
+ * theta the rotation angle; p[] the channel probabilities (3 for
+ * PAULI_ERROR, 1 for DEPOLARIZE1/2, 15 for PAULI_ERROR2). */
+typedef struct ppvm_op {
+  uint32_t code;
+  uint32_t q0;
+  uint32_t q1;
+  uint32_t reserved;
+  double theta;
+  double p[15];
+} ppvm_op;
+
+/* Counters kept by a handle (device side, read back on request). */
+typedef struct ppvm_stats {
+  uint64_t term_updates;   /* sum over gates of live input terms             */
+  uint64_t gates;          /* gates applied (rotations + map ops)            */
+  uint64_t rotations;      /* rotation gates applied                         */
+  uint64_t n_entries;      /* arena entries (live + dead)                    */
+  uint64_t n_live;         /* live terms                                     */
+  uint64_t capacity;       /* arena capacity in entries                      */
+  uint64_t peak_live;      /* largest live count seen at a sync point        */
+  uint64_t syncs;          /* host synchronisations performed                */
+  uint64_t regrows;        /* arena growths (overflow replays included)      */
+  uint64_t compactions;    /* dead-entry compactions                         */
+} ppvm_stats;
+
+typedef struct ppvm_sum* ppvm_handle;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* ppvm_last_error(void);
+
+/* Library / device information: writes a short description (device name,
+ * CU count, gfx arch) into buf. */
+int ppvm_device_info(char* buf, size_t cap);
+
+/* Replaces PauliSum::builder().n_qubits().strategy(CombinedStrategy(
+ *   CoefficientThreshold(min_abs_coeff), MaxPauliWeight(max_pauli_weight)))
+ *   .capacity().build()  (crates/ppvm-pauli-sum/src/sum/data.rs:66-92,
+ *   crates/ppvm-python-native/src/interface.rs:80-111).
+ * max_pauli_weight = UINT64_MAX disables the weight cap (strategy.rs:86-88).
+ * capacity_hint: initial arena size in terms (0 = default). */
+int ppvm_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_pauli_weight,
+                uint64_t capacity_hint, ppvm_handle* out);
+int ppvm_destroy(ppvm_handle h);
+/* Deep copy, replaces Clone / __copy__ (interface.rs:373-387). */
+int ppvm_clone(ppvm_handle h, ppvm_handle* out);
+/* Words per plane W for this handle. */
+int ppvm_words(ppvm_handle h, uint32_t* w);
+/* Preserve set, replaces the `preserve_strings` builder argument
+ * (crates/ppvm-pauli-sum/src/sum/data.rs:75-77). x,z: count*W words. */
+int ppvm_set_preserve(ppvm_handle h, const uint64_t* x, const uint64_t* z, size_t count);
+
+/* Replaces `ps += (word, c)` = entry(k).and_modify(+=).or_insert(c)
+ * (crates/ppvm-pauli-sum/src/sum/ops.rs:140-151, map/hashmap.rs:40-44),
+ * one call per term in order.  x,z: count*W words (term-major). */
+int ppvm_add_terms(ppvm_handle h, const uint64_t* x, const uint64_t* z, const double* c,
+                   size_t count);
+
+/* Clifford gates on a target list; replace Clifford / CliffordExtensions and
+ * their *_many forms (crates/ppvm-traits/src/traits/clifford.rs:30-79,175-218). */
+int ppvm_x(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_y(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_z(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_h(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_s(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_s_dag(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_sqrt_x(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_sqrt_x_dag(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_sqrt_y(ppvm_handle h, const uint32_t* targets, size_t n);
+int ppvm_sqrt_y_dag(ppvm_handle h, const uint32_t* targets, size_t n);
+/* pairs: 2*npairs qubit indices (control, target)... */
+int ppvm_cnot(ppvm_handle h, const uint32_t* pairs, size_t npairs);
+int ppvm_cz(ppvm_handle h, const uint32_t* pairs, size_t npairs);
+int ppvm_cy(ppvm_handle h, const uint32_t* pairs, size_t npairs);
+
+/* Rotations; replace RotationOne::{rx,ry,rz}_many (traits/branch/rot1.rs:8-45)
+ * and RotationTwo::{rxx,ryy,rzz}_many (traits/branch/rot2.rs:6-47).
+ * sin/cos of theta are taken once per call on the host, as the reference does
+ * (sum/rot1.rs:58). */
+int ppvm_rx(ppvm_handle h, const uint32_t* targets, size_t n, double theta);
+int ppvm_ry(ppvm_handle h, const uint32_t* targets, size_t n, double theta);
+int ppvm_rz(ppvm_handle h, const uint32_t* targets, size_t n, double theta);
+int ppvm_rxx(ppvm_handle h, const uint32_t* pairs, size_t npairs, double theta);
+int ppvm_ryy(ppvm_handle h, const uint32_t* pairs, size_t npairs, double theta);
+int ppvm_rzz(ppvm_handle h, const uint32_t* pairs, size_t npairs, double theta);
+/* RotXY::r = rz(phi); rx(theta); rz(-phi)  (sum/rot1.rs:135-143). */
+int ppvm_r(ppvm_handle h, uint32_t q, double axis_angle, double theta);
+
+/* Diagonal noise channels, crates/ppvm-pauli-sum/src/sum/noise.rs:17-131. */
+int ppvm_pauli_error(ppvm_handle h, uint32_t q, const double p[3]);
+int ppvm_depolarize1(ppvm_handle h, uint32_t q, double p);
+int ppvm_depolarize2(ppvm_handle h, uint32_t a, uint32_t b, double p);
+int ppvm_two_qubit_pauli_error(ppvm_handle h, uint32_t a, uint32_t b, const double p[15]);
+
+/* Replaces PauliSum::truncate (crates/ppvm-pauli-sum/src/sum/data.rs:275-311). */
+int ppvm_truncate(ppvm_handle h);
+
+/* Replaces overlap_with_zero = trace("Z?*")
+ * (crates/ppvm-python-native/src/interface.rs:122-130, sum/trace.rs:19-21):
+ * the sum of coefficients of terms whose x plane is all zero. */
+int ppvm_overlap_with_zero(ppvm_handle h, double* out);
+/* Replaces PauliSum::overlap (crates/ppvm-pauli-sum/src/sum/trace.rs:30-37):
+ * sum over terms k of b of b[k] * a[k].  Both handles must have equal n_qubits. */
+int ppvm_overlap(ppvm_handle a, ppvm_handle b, double* out);
+
+/* Replaces len() (sum/data.rs:165). */
+int ppvm_len(ppvm_handle h, uint64_t* out);
+/* Replaces terms() / iter() (interface.rs:393-395): live terms in insertion
+ * order.  If cap < len, nothing is written, *n = len and PPVM_EINVAL is returned. */
+int ppvm_export_terms(ppvm_handle h, uint64_t* x, uint64_t* z, double* c, size_t cap,
+                      size_t* n);
+
+/* Apply a whole program (a sequence of ops, PPVM_OP_TRUNCATE included) in one
+ * call.  Equivalent to the corresponding sequence of single calls above. */
+int ppvm_apply(ppvm_handle h, const ppvm_op* ops, size_t n);
+
+/* Wait for all queued work; check device-side overflow and regrow if needed. */
+int ppvm_synchronize(ppvm_handle h);
+int ppvm_get_stats(ppvm_handle h, ppvm_stats* out);
+
+/* Per-phase kernel timing (HIP events on the handle's stream), for the
+ * measurement harness.  Phases: 0 rotation scan, 1 rotation pair/merge,
+ * 2 rotation emit, 3 map program (Clifford/noise/truncate), 4 overlap,
+ * 5 compaction.  When enabled, every kernel launch is bracketed by events
+ * and its duration accumulated; ms[i] and launches[i] receive the totals. */
+int ppvm_profile_enable(ppvm_handle h, int enable);
+int ppvm_profile_read(ppvm_handle h, double ms[8], uint64_t launches[8]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PPVM_HIP_H */

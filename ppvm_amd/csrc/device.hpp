@@ -1,0 +1,293 @@
+// device.hpp — device-side building blocks of the gfx950 Pauli-propagation engine.
+//
+// Data layout in HBM (per PauliSum handle):
+//   planes : u64[2W][cap]   structure of arrays; plane p < W holds x word p of
+//                           every term, plane W+p holds z word p.
+//   coef   : f64[cap]       coefficient; a truncated ("dead") entry holds the
+//                           NaN pattern DEAD_BITS and is skipped by every kernel.
+//   wl     : WLRec<W>[cap]  per-gate worklist of anticommuting terms (AoS, one
+//                           record = full key + coefficient + arena index).
+//   table  : u64[tcap]      open-addressed pair table (generation-tagged slots).
+//   status : u64[tiles]     decoupled look-back tile status words.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef uint64_t u64;
+typedef uint32_t u32;
+
+namespace ppvm {
+
+constexpr int kBlock = 256;        // threads per block (4 wave64)
+constexpr int kItems = 4;          // entries per thread in tiled kernels
+constexpr int kTile = kBlock * kItems;
+constexpr int kWaves = kBlock / 64;
+
+// Dead-entry marker: a quiet NaN with a fixed payload (sign ignored).
+constexpr u64 DEAD_BITS = 0x7ff8dead0000dea1ull;
+constexpr u64 ABS_MASK = 0x7fffffffffffffffull;
+
+__device__ __forceinline__ bool is_dead(double c) {
+  return (((u64)__double_as_longlong(c)) & ABS_MASK) == DEAD_BITS;
+}
+__device__ __forceinline__ double dead_value() { return __longlong_as_double((long long)DEAD_BITS); }
+
+// Per-op counters, double-buffered by op parity: op k reads slot k&1 and
+// writes slot (k+1)&1, so no kernel ever rewrites a count it iterates over.
+struct alignas(64) CountSlot {
+  u64 n_entries;  // arena entries (live + dead)
+  u64 n_dead;
+  u64 pad[6];
+};
+
+struct alignas(64) DevState {
+  CountSlot slot[2];
+  u64 n_wl;           // worklist length of the current rotation
+  u64 overflow;       // 0, or (op sequence number + 1) of the first overflow
+  u64 op_kills;       // kills of the op in flight (reset by its finaliser)
+  u64 op_updates;     // term updates of the map program in flight
+  u64 term_updates;   // sum over ops of live input terms
+  u64 gates;          // ops applied (rotations + map ops)
+  u64 rotations;
+  u64 scanned;        // entries scanned by rotation scans
+  u64 wl_total;       // anticommuting terms over all rotations
+  u64 pairs_total;    // pairs merged (two terms each)
+  u64 children_total; // new terms appended (after truncation)
+  u64 kills_total;    // terms truncated
+  u64 map_entries;    // entries visited by map programs
+  u64 map_launches;
+  double overlap;     // last overlap_with_zero result
+  u32 ticket[4];      // self-resetting tile tickets (one per tiled kernel kind)
+  u32 done_count[8];  // self-resetting block counters (last-block finalisers)
+  u64 pad[2];
+};
+
+enum TicketKind { TK_SCAN = 0, TK_EMIT = 1, TK_COMPACT = 2 };
+enum DoneKind { DK_SCAN = 0, DK_PAIR = 1, DK_EMIT = 2, DK_MAP = 3, DK_OVERLAP = 4, DK_COMPACT = 5 };
+
+// Rotation gate exp(-i theta/2 G) with G acting on qubits a (and b).
+struct RotSpec {
+  u32 a, b;
+  u32 two;                 // 1 for a two-qubit generator
+  u32 gax, gaz, gbx, gbz;  // generator Pauli bits on a and b
+  u32 trunc;               // fused truncation after the gate
+  double sn, cs;           // sin(theta), cos(theta) from the host libm
+};
+
+struct TruncSpec {
+  double tau;              // drop |c| < tau
+  u64 maxw;                // drop weight > maxw (UINT64_MAX: off)
+  const u64* preserve;     // n_preserve keys of 2W words (never dropped)
+  u32 n_preserve;
+  u32 W;
+};
+
+// ---- map programs (Clifford / noise / truncation), passed by value ---------
+constexpr int kMaxMapOps = 48;
+constexpr int kMaxFactorTables = 8;
+enum MapCode : u32 {
+  M_X = 1, M_Y, M_Z, M_H, M_S, M_S_DAG, M_SQRT_X, M_SQRT_X_DAG, M_SQRT_Y, M_SQRT_Y_DAG,
+  M_CNOT = 11, M_CZ, M_CY,
+  M_SCALE = 30,            // coefficient *= table[4*code_a + code_b] where mask bit set
+};
+struct MapOp {
+  u32 code_flags;          // code | trunc << 8 | table << 16
+  u32 a, b;
+};
+struct MapProgram {
+  u32 n_ops;
+  u32 trunc_before;        // full truncation check before the first op
+  MapOp ops[kMaxMapOps];
+  u32 apply_mask[kMaxFactorTables];   // 16-bit mask of table entries that apply
+  double factors[kMaxFactorTables][16];
+};
+
+// ---- small helpers --------------------------------------------------------
+__device__ __forceinline__ u64 mix64(u64 h) {
+  h ^= h >> 31;
+  h *= 0x9e3779b97f4a7c15ull;
+  h ^= h >> 29;
+  h *= 0xbf58476d1ce4e5b9ull;
+  h ^= h >> 32;
+  return h;
+}
+
+template <int W>
+__device__ __forceinline__ u64 hash_key(const u64 (&k)[2 * W]) {
+  u64 h = 0x6a09e667f3bcc909ull;
+#pragma unroll
+  for (int i = 0; i < 2 * W; ++i) h = mix64(h ^ k[i]) + (u64)i;
+  return mix64(h);
+}
+
+template <int W>
+__device__ __forceinline__ u32 weight_of(const u64 (&k)[2 * W]) {
+  u32 w = 0;
+#pragma unroll
+  for (int i = 0; i < W; ++i) w += __popcll(k[i] | k[W + i]);
+  return w;
+}
+
+template <int W>
+__device__ __forceinline__ bool is_preserved(const u64 (&k)[2 * W], const TruncSpec& t) {
+  for (u32 p = 0; p < t.n_preserve; ++p) {
+    bool eq = true;
+#pragma unroll
+    for (int i = 0; i < 2 * W; ++i) eq &= (t.preserve[p * 2 * W + i] == k[i]);
+    if (eq) return true;
+  }
+  return false;
+}
+
+// Truncation predicate of CombinedStrategy(CoefficientThreshold, MaxPauliWeight)
+// with the preserve set applied on top.
+template <int W>
+__device__ __forceinline__ bool drop_term(const u64 (&k)[2 * W], double c, const TruncSpec& t) {
+  bool drop = fabs(c) < t.tau;
+  if (!drop && t.maxw != ~0ull) drop = weight_of<W>(k) > t.maxw;
+  if (drop && t.n_preserve) drop = !is_preserved<W>(k, t);
+  return drop;
+}
+
+__device__ __forceinline__ u32 bit_of(u64 w, u32 q) { return (u32)(w >> (q & 63)) & 1u; }
+
+// ---- scoped atomics (agent scope: visible across all XCDs) -----------------
+__device__ __forceinline__ u64 ld_acquire(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(u64* p, u64 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 ld_relaxed(const u64* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u64 cas_u64(u64* p, u64 expected, u64 desired) {
+  __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+  return expected;  // value observed in memory
+}
+__device__ __forceinline__ u64 add_u64(u64* p, u64 v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ u32 add_u32(u32* p, u32 v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_u32(u32* p, u32 v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- tiles: in-order tickets + decoupled look-back -------------------------
+// Status word: gen (24 bits) | flag (2 bits) | value (38 bits).
+constexpr u64 ST_AGG = 1, ST_INCL = 2;
+__device__ __forceinline__ u64 st_pack(u32 gen, u64 flag, u64 v) {
+  return ((u64)gen << 40) | (flag << 38) | (v & ((1ull << 38) - 1));
+}
+
+// Takes the next tile index; tiles are handed out in increasing order so a
+// tile only ever waits on tiles that are already running.  Returns -1 when all
+// tiles are taken; the block taking the very last ticket resets the counter
+// for the next launch.
+__device__ __forceinline__ long long next_tile(u32* ticket, u64 n_tiles, u32* lds_slot) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u32 t = add_u32(ticket, 1u);
+    if ((u64)t >= n_tiles) {
+      if ((u64)t == n_tiles + gridDim.x - 1) st_u32(ticket, 0u);
+      *lds_slot = 0xffffffffu;
+    } else {
+      *lds_slot = t;
+    }
+  }
+  __syncthreads();
+  u32 v = *lds_slot;
+  return v == 0xffffffffu ? -1 : (long long)v;
+}
+
+// Exclusive prefix of this tile over all earlier tiles (thread 0 computes,
+// result returned to every thread through lds).
+__device__ __forceinline__ u64 lookback(u64* status, u32 gen, u64 tile, u64 agg, u64* lds_u64) {
+  if (threadIdx.x == 0) {
+    u64 prefix = 0;
+    if (tile == 0) {
+      st_release(&status[0], st_pack(gen, ST_INCL, agg));
+    } else {
+      st_release(&status[tile], st_pack(gen, ST_AGG, agg));
+      long long j = (long long)tile - 1;
+      while (true) {
+        u64 s = ld_acquire(&status[j]);
+        u64 flag = (s >> 38) & 3ull;
+        if ((u32)(s >> 40) != gen || flag == 0) continue;  // predecessor not yet published
+        prefix += s & ((1ull << 38) - 1);
+        if (flag == ST_INCL) break;
+        --j;
+      }
+      st_release(&status[tile], st_pack(gen, ST_INCL, prefix + agg));
+    }
+    *lds_u64 = prefix;
+  }
+  __syncthreads();
+  return *lds_u64;
+}
+
+// Block-wide ranks of 0/1 flags over kItems rows: element (row i, thread t)
+// precedes (row i', thread t') iff (i, t) < (i', t').  pos[i] is the rank of
+// this thread's element in row i among flagged elements; returns the total.
+__device__ __forceinline__ u32 tile_ranks(const bool (&f)[kItems], u32 (&pos)[kItems], u32* lds_cnt) {
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  u32 inwave[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    u64 b = __ballot(f[i]);
+    inwave[i] = (u32)__popcll(b & ((1ull << lane) - 1ull));
+    if (lane == 0) lds_cnt[i * kWaves + wave] = (u32)__popcll(b);
+  }
+  __syncthreads();
+  u32 run = 0;
+  u32 base[kItems];
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+#pragma unroll
+    for (int w = 0; w < kWaves; ++w) {
+      u32 v = lds_cnt[i * kWaves + w];
+      if ((u32)w == wave) base[i] = run;
+      run += v;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) pos[i] = base[i] + inwave[i];
+  return run;
+}
+
+// Last-block detection: every block calls this once after its work; exactly
+// one block (the last to arrive) gets true and must reset nothing itself —
+// the counter is reset here for the next launch.
+__device__ __forceinline__ bool last_block(u32* done, u32* lds_flag) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    u32 d = add_u32(done, 1u);
+    bool last = d == gridDim.x - 1;
+    if (last) st_u32(done, 0u);
+    *lds_flag = last ? 1u : 0u;
+  }
+  __syncthreads();
+  bool r = *lds_flag != 0u;
+  if (r) __threadfence();
+  return r;
+}
+
+// Deterministic block sum of a u64 (for counters).
+__device__ __forceinline__ u64 block_sum_u64(u64 v, u64* lds) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  u64 t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kWaves; ++w) t += lds[w];
+  return t;  // valid in thread 0
+}
+
+}  // namespace ppvm

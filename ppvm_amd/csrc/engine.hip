@@ -1,0 +1,1102 @@
+// engine.hip — host implementation of the C ABI in include/ppvm_hip.h.
+//
+// A handle owns one HIP stream and its device arenas.  Gate calls are queued
+// on the host and issued lazily so that (i) a truncation that directly follows
+// a gate is fused into that gate's kernels, and (ii) runs of Clifford / noise /
+// truncation ops become one k_map launch.  Kernels keep their counts in device
+// memory (DevState), so no gate needs a host round trip; the host synchronises
+// every kSyncEvery issued ops (or when asked for a result), checks the device
+// overflow flag, and if a gate ran out of arena capacity it grows the arena and
+// re-issues the gates from the one that overflowed (gates that see the flag do
+// nothing, so the arena still holds that gate's input).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ppvm_hip.h"
+#include "kernels.hpp"
+
+using namespace ppvm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess) return fail(PPVM_EHIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr u64 kMinCap = 4096;
+constexpr u64 kMaxCap = 1ull << 31;  // worklist indices are 32-bit
+constexpr int kSyncEvery = 24;
+constexpr int kMaxGrid = 1024;
+constexpr int kOverlapGrid = 512;
+constexpr u32 kGenLimit = (1u << 24) - 2;
+
+enum IssuedKind { IK_ROT = 1, IK_MAP = 2 };
+struct Issued {
+  int kind;
+  u64 seq;
+  RotSpec rot;
+  MapProgram map;
+};
+
+u64 pow2_at_least(u64 v) {
+  u64 p = 1;
+  while (p < v) p <<= 1;
+  return p;
+}
+
+bool is_rotation(u32 c) { return c >= PPVM_OP_RX && c <= PPVM_OP_RZZ; }
+bool is_two_qubit(u32 c) {
+  return c == PPVM_OP_CNOT || c == PPVM_OP_CZ || c == PPVM_OP_CY || c == PPVM_OP_RXX ||
+         c == PPVM_OP_RYY || c == PPVM_OP_RZZ || c == PPVM_OP_DEPOLARIZE2 ||
+         c == PPVM_OP_PAULI_ERROR2;
+}
+bool is_map_op(u32 c) {
+  return (c >= PPVM_OP_X && c <= PPVM_OP_CY) || (c >= PPVM_OP_PAULI_ERROR && c <= PPVM_OP_PAULI_ERROR2);
+}
+// ops after which a truncated state stays truncated (sign flips only)
+bool keeps_clean(u32 c) { return c == PPVM_OP_X || c == PPVM_OP_Y || c == PPVM_OP_Z; }
+
+// two_qubit_pauli_error: indices of the channel Paulis
+// (IX,IY,IZ,XI,XX,XY,XZ,YI,YX,YY,YZ,ZI,ZX,ZY,ZZ) anticommuting with (P_a,P_b),
+// in the summation order of crates/ppvm-pauli-sum/src/sum/noise.rs:56-106
+// (rows in the order I, X, Y, Z of P_a then P_b).  The order of the FP
+// subtractions fixes the bits, so it is kept as in the reference.
+const int kPe2Rows[16][8] = {
+    {-1, -1, -1, -1, -1, -1, -1, -1}, {1, 10, 13, 14, 2, 5, 6, 9},   {0, 10, 12, 14, 2, 4, 6, 8},
+    {0, 1, 12, 13, 4, 5, 8, 9},       {10, 11, 12, 13, 14, 7, 8, 9}, {1, 11, 12, 2, 5, 6, 7, 8},
+    {0, 11, 13, 2, 4, 6, 7, 9},       {0, 1, 10, 11, 14, 4, 5, 7},   {11, 12, 13, 14, 3, 4, 5, 6},
+    {1, 10, 11, 12, 2, 3, 4, 9},      {0, 10, 11, 13, 2, 3, 5, 8},   {0, 1, 11, 14, 3, 6, 8, 9},
+    {10, 3, 4, 5, 6, 7, 8, 9},        {1, 13, 14, 2, 3, 4, 7, 8},    {0, 12, 14, 2, 3, 5, 7, 9},
+    {0, 1, 10, 12, 13, 3, 6, 7},
+};
+// Pauli code x|z<<1 (I0 X1 Z2 Y3) -> letter index I0 X1 Y2 Z3 of the rows above.
+const int kCodeToLetter[4] = {0, 1, 3, 2};
+
+}  // namespace
+
+struct ppvm_sum {
+  u32 n = 0, W = 1;  // W: words per plane inside the engine (1, 2, 4 or 8)
+  u32 Wa = 1;        // words per plane at the ABI: ceil(n/64)
+  double tau = 0.0;
+  u64 maxw = ~0ull;
+  bool trunc_active = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  u64* planes[2] = {nullptr, nullptr};
+  double* coef[2] = {nullptr, nullptr};
+  int cur = 0;
+  u64 cap = 0;
+  void* wl = nullptr;
+  u64* table = nullptr;
+  u64 tcap = 0;
+  u64* status = nullptr;
+  u64 n_status = 0;
+  double* partials = nullptr;
+  DevState* st = nullptr;
+  DevState* st_host = nullptr;  // pinned
+  u64* preserve = nullptr;
+  u32 n_preserve = 0;
+  std::vector<u64> preserve_host;
+  u64 op_seq = 0;
+  u32 lb_gen = 0;
+  u32 tab_gen = 0;
+  bool clean = true;
+  bool broken = false;
+  std::vector<ppvm_op> pending;
+  std::vector<Issued> log;
+  int issued_since_sync = 0;
+  u64 n_bound = 0;  // host-side upper bound of arena entries (grid sizing)
+  ppvm_stats hs{};
+  // profiling
+  bool prof = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<int> ev_phase;
+  size_t ev_used = 0;
+  double prof_ms[8] = {0};
+  u64 prof_n[8] = {0};
+
+  size_t wl_rec() const { return 16 * (size_t)W + 16; }
+  TruncSpec trunc_spec() const {
+    TruncSpec t;
+    t.tau = tau;
+    t.maxw = maxw;
+    t.preserve = preserve;
+    t.n_preserve = n_preserve;
+    t.W = W;
+    return t;
+  }
+};
+
+namespace {
+
+// ---- profiling helpers ---------------------------------------------------------
+int prof_begin(ppvm_sum* s, int phase) {
+  if (!s->prof) return PPVM_OK;
+  while (s->ev.size() < s->ev_used + 2) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    s->ev.push_back(e);
+  }
+  s->ev_phase.resize(s->ev.size() / 2 + 1);
+  s->ev_phase[s->ev_used / 2] = phase;
+  HIPCHK(hipEventRecord(s->ev[s->ev_used], s->stream));
+  return PPVM_OK;
+}
+int prof_end(ppvm_sum* s) {
+  if (!s->prof) return PPVM_OK;
+  HIPCHK(hipEventRecord(s->ev[s->ev_used + 1], s->stream));
+  s->ev_used += 2;
+  return PPVM_OK;
+}
+int prof_collect(ppvm_sum* s) {
+  for (size_t i = 0; i + 1 < s->ev_used; i += 2) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev[i], s->ev[i + 1]));
+    int ph = s->ev_phase[i / 2];
+    s->prof_ms[ph] += ms;
+    s->prof_n[ph] += 1;
+  }
+  s->ev_used = 0;
+  return PPVM_OK;
+}
+
+int grid_for(u64 n, u64 per_block, int maxg) {
+  u64 g = (n + per_block - 1) / per_block;
+  if (g < 1) g = 1;
+  if (g > (u64)maxg) g = maxg;
+  return (int)g;
+}
+
+u32 next_lb_gen(ppvm_sum* s, int* rc) {
+  *rc = PPVM_OK;
+  if (s->lb_gen >= kGenLimit) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s->stream, s->status, s->n_status, 0ull);
+    s->lb_gen = 0;
+  }
+  return ++s->lb_gen;
+}
+u32 next_tab_gen(ppvm_sum* s) {
+  if (s->tab_gen >= kGenLimit) {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s->stream, s->table, s->tcap, 0ull);
+    s->tab_gen = 0;
+  }
+  return ++s->tab_gen;
+}
+
+// ---- allocation -------------------------------------------------------------------
+void free_buffers(ppvm_sum* s) {
+  for (int i = 0; i < 2; ++i) {
+    if (s->planes[i]) (void)hipFree(s->planes[i]);
+    if (s->coef[i]) (void)hipFree(s->coef[i]);
+    s->planes[i] = nullptr;
+    s->coef[i] = nullptr;
+  }
+  if (s->wl) (void)hipFree(s->wl);
+  if (s->table) (void)hipFree(s->table);
+  if (s->status) (void)hipFree(s->status);
+  s->wl = nullptr;
+  s->table = nullptr;
+  s->status = nullptr;
+}
+
+// (Re)allocate all capacity-dependent buffers for capacity new_cap, keeping the
+// first n_keep entries of the current arena.
+int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
+  if (new_cap > kMaxCap) return fail(PPVM_ENOMEM, "arena capacity %llu exceeds the limit of %llu terms",
+                                     (unsigned long long)new_cap, (unsigned long long)kMaxCap);
+  const u32 np = 2 * s->W;
+  u64* np0 = nullptr;
+  double* nc0 = nullptr;
+  u64* np1 = nullptr;
+  double* nc1 = nullptr;
+  if (hipMalloc(&np0, new_cap * np * 8) != hipSuccess || hipMalloc(&nc0, new_cap * 8) != hipSuccess ||
+      hipMalloc(&np1, new_cap * np * 8) != hipSuccess || hipMalloc(&nc1, new_cap * 8) != hipSuccess) {
+    if (np0) (void)hipFree(np0);
+    if (nc0) (void)hipFree(nc0);
+    if (np1) (void)hipFree(np1);
+    if (nc1) (void)hipFree(nc1);
+    (void)hipGetLastError();
+    return fail(PPVM_ENOMEM, "device allocation of %llu-term arenas failed", (unsigned long long)new_cap);
+  }
+  if (n_keep && s->planes[s->cur]) {
+    hipLaunchKernelGGL(k_copy_arena, dim3(grid_for(n_keep, 256, 4096)), dim3(256), 0, s->stream,
+                       s->planes[s->cur], s->coef[s->cur], s->cap, np0, nc0, new_cap, np, n_keep);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < 2; ++i) {
+    if (s->planes[i]) (void)hipFree(s->planes[i]);
+    if (s->coef[i]) (void)hipFree(s->coef[i]);
+  }
+  s->planes[0] = np0;
+  s->coef[0] = nc0;
+  s->planes[1] = np1;
+  s->coef[1] = nc1;
+  s->cur = 0;
+  if (s->wl) (void)hipFree(s->wl);
+  if (s->table) (void)hipFree(s->table);
+  if (s->status) (void)hipFree(s->status);
+  s->wl = nullptr;
+  s->table = nullptr;
+  s->status = nullptr;
+  s->tcap = pow2_at_least(2 * new_cap);
+  s->n_status = new_cap / kTile + 2;
+  if (hipMalloc(&s->wl, new_cap * s->wl_rec()) != hipSuccess ||
+      hipMalloc(&s->table, s->tcap * 8) != hipSuccess ||
+      hipMalloc(&s->status, s->n_status * 8) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PPVM_ENOMEM, "device allocation of work buffers for %llu terms failed",
+                (unsigned long long)new_cap);
+  }
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(s->tcap, 256, 4096)), dim3(256), 0, s->stream, s->table,
+                     s->tcap, 0ull);
+  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(s->n_status, 256, 1024)), dim3(256), 0, s->stream,
+                     s->status, s->n_status, 0ull);
+  HIPCHK(hipGetLastError());
+  s->lb_gen = 0;
+  s->tab_gen = 0;
+  s->cap = new_cap;
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return PPVM_OK;
+}
+
+// ---- kernel launches, dispatched on W ------------------------------------------
+template <int W>
+int launch_rot(ppvm_sum* s, const Issued& it) {
+  const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
+  int rc;
+  const int grid = grid_for(s->n_bound, kTile, kMaxGrid);
+  WLRec<W>* wl = reinterpret_cast<WLRec<W>*>(s->wl);
+  u32 g1 = next_lb_gen(s, &rc);
+  if ((rc = prof_begin(s, 0))) return rc;
+  hipLaunchKernelGGL(k_rot_scan<W>, dim3(grid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
+                     s->coef[s->cur], s->cap, wl, s->st, in, out, it.rot, s->status, g1);
+  if ((rc = prof_end(s))) return rc;
+  u32 tg = next_tab_gen(s);
+  if ((rc = prof_begin(s, 1))) return rc;
+  hipLaunchKernelGGL(k_rot_pair<W>, dim3(grid_for(s->n_bound, kBlock * 2, 4 * kMaxGrid)), dim3(kBlock), 0,
+                     s->stream, s->coef[s->cur], s->cap, wl, s->st, in, it.rot, s->table, s->tcap, tg,
+                     s->trunc_spec(), it.seq + 1);
+  if ((rc = prof_end(s))) return rc;
+  u32 g2 = next_lb_gen(s, &rc);
+  if ((rc = prof_begin(s, 2))) return rc;
+  hipLaunchKernelGGL(k_rot_emit<W>, dim3(grid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
+                     s->coef[s->cur], s->cap, wl, s->st, in, out, it.rot, s->trunc_spec(), s->status, g2);
+  if ((rc = prof_end(s))) return rc;
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
+template <int W>
+int launch_map(ppvm_sum* s, const Issued& it) {
+  const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
+  int rc;
+  if ((rc = prof_begin(s, 3))) return rc;
+  hipLaunchKernelGGL(k_map<W>, dim3(grid_for(s->n_bound, kBlock * 4, 4 * kMaxGrid)), dim3(kBlock), 0,
+                     s->stream, s->planes[s->cur], s->coef[s->cur], s->cap, s->st, in, out, it.map,
+                     s->trunc_spec());
+  if ((rc = prof_end(s))) return rc;
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
+template <int W>
+int launch_overlap(ppvm_sum* s) {
+  int rc;
+  if ((rc = prof_begin(s, 4))) return rc;
+  hipLaunchKernelGGL(k_overlap<W>, dim3(kOverlapGrid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
+                     s->coef[s->cur], s->cap, s->st, (u32)(s->op_seq & 1), s->partials);
+  if ((rc = prof_end(s))) return rc;
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
+template <int W>
+int launch_compact(ppvm_sum* s) {
+  int rc;
+  const u32 in = (u32)(s->op_seq & 1), out = in ^ 1u;
+  u32 gen = next_lb_gen(s, &rc);
+  if ((rc = prof_begin(s, 5))) return rc;
+  hipLaunchKernelGGL(k_compact<W>, dim3(grid_for(s->n_bound, kTile, kMaxGrid)), dim3(kBlock), 0, s->stream,
+                     s->planes[s->cur], s->coef[s->cur], s->planes[s->cur ^ 1], s->coef[s->cur ^ 1], s->cap,
+                     s->st, in, out, s->status, gen);
+  if ((rc = prof_end(s))) return rc;
+  HIPCHK(hipGetLastError());
+  s->op_seq += 1;
+  s->cur ^= 1;
+  return PPVM_OK;
+}
+
+#define PPVM_DISPATCH_W(s, fn, ...)                  \
+  switch ((s)->W) {                                  \
+    case 1: return fn<1>(s, ##__VA_ARGS__);          \
+    case 2: return fn<2>(s, ##__VA_ARGS__);          \
+    case 4: return fn<4>(s, ##__VA_ARGS__);          \
+    case 8: return fn<8>(s, ##__VA_ARGS__);          \
+    default: return fail(PPVM_EINVAL, "unsupported word count %u", (s)->W); \
+  }
+
+int issue_rot_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_rot, it); }
+int issue_map_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_map, it); }
+int overlap_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_overlap); }
+int compact_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_compact); }
+
+int issue(ppvm_sum* s, Issued& it, bool log_it) {
+  int rc = it.kind == IK_ROT ? issue_rot_w(s, it) : issue_map_w(s, it);
+  if (rc) {
+    s->broken = true;
+    return rc;
+  }
+  if (it.kind == IK_ROT) {
+    s->n_bound = std::min<u64>(s->cap, s->n_bound * 2 + 1);
+  }
+  if (log_it) s->log.push_back(it);
+  return PPVM_OK;
+}
+
+int read_state(ppvm_sum* s) {
+  HIPCHK(hipMemcpyAsync(s->st_host, s->st, sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->hs.syncs += 1;
+  return prof_collect(s);
+}
+
+// Synchronise; on overflow grow and replay.  Afterwards the log is empty and
+// st_host holds the state after the last issued op.
+int sync_check(ppvm_sum* s) {
+  if (s->broken) return fail(PPVM_ESTATE, "handle is unusable after an earlier device error");
+  int rc;
+  for (int round = 0; round < 64; ++round) {
+    if ((rc = read_state(s))) {
+      s->broken = true;
+      return rc;
+    }
+    const u64 of = s->st_host->overflow;
+    if (!of) break;
+    const u64 seq = of - 1;
+    const CountSlot& c = s->st_host->slot[seq & 1];
+    const u64 live = c.n_entries - c.n_dead;
+    u64 want = pow2_at_least(std::max<u64>(2 * s->cap, c.n_entries + 2 * live + 1));
+    if ((rc = resize(s, want, c.n_entries))) {
+      s->broken = true;
+      return rc;
+    }
+    s->hs.regrows += 1;
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, &s->st->overflow, 1ull, 0ull);
+    HIPCHK(hipGetLastError());
+    s->n_bound = c.n_entries;
+    std::vector<Issued> redo;
+    for (auto& it : s->log)
+      if (it.seq >= seq) redo.push_back(it);
+    for (auto& it : redo)
+      if ((rc = issue(s, it, false))) return rc;
+  }
+  if (s->st_host->overflow) {
+    s->broken = true;
+    return fail(PPVM_ENOMEM, "arena overflow persisted after repeated growth");
+  }
+  s->log.clear();
+  s->issued_since_sync = 0;
+  const CountSlot& c = s->st_host->slot[s->op_seq & 1];
+  s->n_bound = c.n_entries;
+  const u64 live = c.n_entries - c.n_dead;
+  s->hs.peak_live = std::max(s->hs.peak_live, live);
+  // compaction when dead entries dominate the scans
+  if (c.n_dead > 4096 && c.n_dead * 4 > c.n_entries) {
+    if ((rc = compact_w(s))) {
+      s->broken = true;
+      return rc;
+    }
+    s->hs.compactions += 1;
+    if ((rc = read_state(s))) {
+      s->broken = true;
+      return rc;
+    }
+    s->n_bound = s->st_host->slot[s->op_seq & 1].n_entries;
+  }
+  // proactive growth: keep room for a full gate of children
+  const CountSlot& c2 = s->st_host->slot[s->op_seq & 1];
+  const u64 live2 = c2.n_entries - c2.n_dead;
+  if (c2.n_entries + 2 * live2 > s->cap) {
+    u64 want = pow2_at_least(c2.n_entries + 2 * live2 + 1);
+    if ((rc = resize(s, want, c2.n_entries))) {
+      s->broken = true;
+      return rc;
+    }
+    s->hs.regrows += 1;
+  }
+  return PPVM_OK;
+}
+
+// ---- op translation -----------------------------------------------------------
+RotSpec make_rot(const ppvm_op& op, bool trunc) {
+  RotSpec g{};
+  g.a = op.q0;
+  g.b = op.q1;
+  g.two = 0;
+  switch (op.code) {
+    case PPVM_OP_RX: g.gax = 1; break;
+    case PPVM_OP_RY: g.gax = 1; g.gaz = 1; break;
+    case PPVM_OP_RZ: g.gaz = 1; break;
+    case PPVM_OP_RXX: g.two = 1; g.gax = 1; g.gbx = 1; break;
+    case PPVM_OP_RYY: g.two = 1; g.gax = g.gaz = g.gbx = g.gbz = 1; break;
+    case PPVM_OP_RZZ: g.two = 1; g.gaz = 1; g.gbz = 1; break;
+  }
+  if (!g.two) g.b = g.a;
+  g.trunc = trunc ? 1u : 0u;
+  // Float::sin_cos of the reference (crates/ppvm-traits/src/traits/coefficient.rs:55-57):
+  // the host libm, once per gate.
+  g.sn = std::sin(op.theta);
+  g.cs = std::cos(op.theta);
+  return g;
+}
+
+// Adds one map op (and its factor table) to prog; returns false if full.
+bool map_add(MapProgram& p, u32& ntables, const ppvm_op& op, bool trunc) {
+  if (p.n_ops >= (u32)kMaxMapOps) return false;
+  const bool noise = op.code >= PPVM_OP_PAULI_ERROR && op.code <= PPVM_OP_PAULI_ERROR2;
+  if (noise && ntables >= (u32)kMaxFactorTables) return false;
+  MapOp m{};
+  m.a = op.q0;
+  m.b = op.q1;
+  u32 code = 0, table = 0;
+  if (!noise) {
+    code = op.code;  // Clifford op codes coincide with MapCode values
+  } else {
+    code = M_SCALE;
+    table = ntables++;
+    double* f = p.factors[table];
+    u32 mask = 0;
+    for (int i = 0; i < 16; ++i) f[i] = 1.0;
+    if (op.code == PPVM_OP_PAULI_ERROR) {  // noise.rs:17-39
+      const double one = 1.0;
+      const double xf = one - op.p[1] * 2.0 - op.p[2] * 2.0;
+      const double zf = one - op.p[0] * 2.0 - op.p[1] * 2.0;
+      const double yf = one - op.p[0] * 2.0 - op.p[2] * 2.0;
+      f[1 * 4] = xf;  // code X = 1
+      f[2 * 4] = zf;  // code Z = 2
+      f[3 * 4] = yf;  // code Y = 3
+      mask = (1u << 4) | (1u << 8) | (1u << 12);
+      m.b = 0xffffffffu;
+    } else if (op.code == PPVM_OP_DEPOLARIZE1) {  // noise.rs:109-118
+      const double fac = 1.0 - op.p[0] * (4.0 / 3.0);
+      for (int ca = 1; ca < 4; ++ca) {
+        f[ca * 4] = fac;
+        mask |= 1u << (ca * 4);
+      }
+      m.b = 0xffffffffu;
+    } else if (op.code == PPVM_OP_DEPOLARIZE2) {  // noise.rs:121-131
+      const double fac = 1.0 - op.p[0] * (16.0 / 15.0);
+      for (int i = 1; i < 16; ++i) {
+        f[i] = fac;
+        mask |= 1u << i;
+      }
+    } else {  // PAULI_ERROR2, noise.rs:52-107
+      for (int ca = 0; ca < 4; ++ca)
+        for (int cb = 0; cb < 4; ++cb) {
+          const int row = 4 * kCodeToLetter[ca] + kCodeToLetter[cb];
+          if (row == 0) continue;
+          double acc = 1.0;
+          for (int j = 0; j < 8; ++j) acc = acc - op.p[kPe2Rows[row][j]] * 2.0;
+          f[ca * 4 + cb] = acc;
+          mask |= 1u << (ca * 4 + cb);
+        }
+    }
+    p.apply_mask[table] = mask;
+  }
+  m.code_flags = code | ((trunc ? 1u : 0u) << 8) | (table << 16);
+  p.ops[p.n_ops++] = m;
+  return true;
+}
+
+int issue_and_maybe_sync(ppvm_sum* s, Issued& it) {
+  it.seq = s->op_seq++;
+  int rc = issue(s, it, true);
+  if (rc) return rc;
+  if (++s->issued_since_sync >= kSyncEvery) return sync_check(s);
+  return PPVM_OK;
+}
+
+// Turns queued ops into kernel launches.  With final=false, a trailing op whose
+// fusion partner may still arrive (a rotation that a truncate could follow, a
+// run of map ops) is kept queued.
+int pump(ppvm_sum* s, bool final) {
+  if (s->broken) return fail(PPVM_ESTATE, "handle is unusable after an earlier device error");
+  size_t i = 0;
+  auto& q = s->pending;
+  int rc = PPVM_OK;
+  while (i < q.size()) {
+    const ppvm_op& op = q[i];
+    if (op.code == PPVM_OP_NOP) {
+      ++i;
+      continue;
+    }
+    if (is_rotation(op.code)) {
+      if (i + 1 >= q.size() && !final) break;
+      const bool trunc_next = i + 1 < q.size() && q[i + 1].code == PPVM_OP_TRUNCATE;
+      const bool fused = trunc_next && s->clean && s->trunc_active;
+      Issued it{};
+      it.kind = IK_ROT;
+      it.rot = make_rot(op, fused);
+      if ((rc = issue_and_maybe_sync(s, it))) return rc;
+      i += fused ? 2 : 1;
+      if (!fused && s->trunc_active) s->clean = false;
+      if (fused) {  // drop repeated truncates (idempotent)
+        while (i < q.size() && q[i].code == PPVM_OP_TRUNCATE) ++i;
+      }
+      continue;
+    }
+    // a run of map ops and truncates
+    size_t j = i;
+    bool ends_queue = true;
+    while (j < q.size()) {
+      if (is_rotation(q[j].code)) {
+        ends_queue = false;
+        break;
+      }
+      ++j;
+    }
+    if (ends_queue && !final) {
+      // wait for more ops unless the run already fills a program
+      size_t nmap = 0;
+      for (size_t k = i; k < j; ++k) nmap += is_map_op(q[k].code);
+      if (nmap < (size_t)kMaxMapOps) break;
+    }
+    Issued it{};
+    it.kind = IK_MAP;
+    MapProgram& p = it.map;
+    std::memset(&p, 0, sizeof(p));
+    u32 ntables = 0;
+    size_t k = i;
+    bool clean = s->clean || !s->trunc_active;
+    bool any = false;
+    while (k < j && q[k].code == PPVM_OP_TRUNCATE) {
+      if (s->trunc_active && !clean) p.trunc_before = 1;
+      clean = true;
+      ++k;
+    }
+    while (k < j) {
+      const ppvm_op& m = q[k];
+      if (m.code == PPVM_OP_NOP) {
+        ++k;
+        continue;
+      }
+      const bool tnext = k + 1 < q.size() && q[k + 1].code == PPVM_OP_TRUNCATE && s->trunc_active;
+      if (!map_add(p, ntables, m, tnext)) break;  // program full: issue and continue
+      any = true;
+      ++k;
+      if (tnext) {
+        clean = true;
+        while (k < j && q[k].code == PPVM_OP_TRUNCATE) ++k;
+      } else if (s->trunc_active && !keeps_clean(m.code)) {
+        clean = false;
+      }
+    }
+    if (any || p.trunc_before) {
+      if ((rc = issue_and_maybe_sync(s, it))) return rc;
+    }
+    s->clean = clean;
+    i = k;
+  }
+  q.erase(q.begin(), q.begin() + (long)i);
+  return PPVM_OK;
+}
+
+int enqueue(ppvm_sum* s, const ppvm_op& op) {
+  s->pending.push_back(op);
+  if (s->pending.size() >= 4 * (size_t)kMaxMapOps) return pump(s, false);
+  return PPVM_OK;
+}
+
+int validate(const ppvm_sum* s, const ppvm_op& op) {
+  const u32 c = op.code;
+  const bool known = c == PPVM_OP_NOP || c == PPVM_OP_TRUNCATE || is_rotation(c) || is_map_op(c);
+  if (!known) return fail(PPVM_EINVAL, "unknown op code %u", c);
+  if (c == PPVM_OP_NOP || c == PPVM_OP_TRUNCATE) return PPVM_OK;
+  if (op.q0 >= s->n) return fail(PPVM_EINVAL, "qubit index %u out of range for %u qubits", op.q0, s->n);
+  if (is_two_qubit(c)) {
+    if (op.q1 >= s->n) return fail(PPVM_EINVAL, "qubit index %u out of range for %u qubits", op.q1, s->n);
+    if (op.q1 == op.q0) return fail(PPVM_EINVAL, "two-qubit op on identical qubits (%u, %u)", op.q0, op.q1);
+  }
+  if (is_rotation(c) && !std::isfinite(op.theta)) return fail(PPVM_EINVAL, "rotation angle is not finite");
+  return PPVM_OK;
+}
+
+int push_gate(ppvm_sum* s, u32 code, u32 q0, u32 q1, double theta, const double* p, int np) {
+  ppvm_op op{};
+  op.code = code;
+  op.q0 = q0;
+  op.q1 = q1;
+  op.theta = theta;
+  for (int i = 0; i < np; ++i) op.p[i] = p[i];
+  int rc = validate(s, op);
+  if (rc) return rc;
+  return enqueue(s, op);
+}
+
+int gate_1q(ppvm_handle h, u32 code, const uint32_t* t, size_t n, double theta) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  if (n && !t) return fail(PPVM_EINVAL, "null target array");
+  for (size_t i = 0; i < n; ++i)
+    if (t[i] >= h->n) return fail(PPVM_EINVAL, "qubit index %u out of range for %u qubits", t[i], h->n);
+  for (size_t i = 0; i < n; ++i) {
+    int rc = push_gate(h, code, t[i], 0, theta, nullptr, 0);
+    if (rc) return rc;
+  }
+  return PPVM_OK;
+}
+int gate_2q(ppvm_handle h, u32 code, const uint32_t* pr, size_t np, double theta) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  if (np && !pr) return fail(PPVM_EINVAL, "null pair array");
+  for (size_t i = 0; i < np; ++i) {
+    if (pr[2 * i] >= h->n || pr[2 * i + 1] >= h->n)
+      return fail(PPVM_EINVAL, "qubit pair (%u, %u) out of range for %u qubits", pr[2 * i], pr[2 * i + 1], h->n);
+    if (pr[2 * i] == pr[2 * i + 1]) return fail(PPVM_EINVAL, "two-qubit gate on identical qubits (%u)", pr[2 * i]);
+  }
+  for (size_t i = 0; i < np; ++i) {
+    int rc = push_gate(h, code, pr[2 * i], pr[2 * i + 1], theta, nullptr, 0);
+    if (rc) return rc;
+  }
+  return PPVM_OK;
+}
+
+// Host copies of the live terms (arena order).
+int download_live(ppvm_sum* s, std::vector<u64>& keys, std::vector<double>& cs) {
+  int rc;
+  if ((rc = pump(s, true))) return rc;
+  if ((rc = sync_check(s))) return rc;
+  const CountSlot c = s->st_host->slot[s->op_seq & 1];
+  const u64 n = c.n_entries;
+  const u32 np = 2 * s->W;
+  std::vector<u64> pl(n * np);
+  std::vector<double> cf(n);
+  if (n) {
+    for (u32 p = 0; p < np; ++p)
+      HIPCHK(hipMemcpyAsync(pl.data() + (size_t)p * n, s->planes[s->cur] + (size_t)p * s->cap, n * 8,
+                            hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(cf.data(), s->coef[s->cur], n * 8, hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  keys.clear();
+  cs.clear();
+  keys.reserve((n - c.n_dead) * np);
+  cs.reserve(n - c.n_dead);
+  for (u64 i = 0; i < n; ++i) {
+    u64 bits;
+    std::memcpy(&bits, &cf[i], 8);
+    if ((bits & ABS_MASK) == DEAD_BITS) continue;
+    for (u32 p = 0; p < np; ++p) keys.push_back(pl[(size_t)p * n + i]);
+    cs.push_back(cf[i]);
+  }
+  return PPVM_OK;
+}
+
+// Replace the arena contents with the given terms (term-major keys).
+int upload(ppvm_sum* s, const std::vector<u64>& keys, const std::vector<double>& cs) {
+  int rc;
+  const u64 n = cs.size();
+  const u32 np = 2 * s->W;
+  if (n + 2 * n > s->cap) {
+    if ((rc = resize(s, pow2_at_least(std::max<u64>(kMinCap, 4 * n)), 0))) return rc;
+  }
+  std::vector<u64> pl((size_t)n * np);
+  for (u64 i = 0; i < n; ++i)
+    for (u32 p = 0; p < np; ++p) pl[(size_t)p * n + i] = keys[i * np + p];
+  if (n) {
+    for (u32 p = 0; p < np; ++p)
+      HIPCHK(hipMemcpyAsync(s->planes[s->cur] + (size_t)p * s->cap, pl.data() + (size_t)p * n, n * 8,
+                            hipMemcpyHostToDevice, s->stream));
+    HIPCHK(hipMemcpyAsync(s->coef[s->cur], cs.data(), n * 8, hipMemcpyHostToDevice, s->stream));
+  }
+  CountSlot c{};
+  c.n_entries = n;
+  c.n_dead = 0;
+  HIPCHK(hipMemcpyAsync(&s->st->slot[s->op_seq & 1], &c, sizeof(CountSlot), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->n_bound = n;
+  return PPVM_OK;
+}
+
+struct KeyHash {
+  size_t W2;
+  size_t operator()(const std::vector<u64>& k) const {
+    u64 h = 0x9e3779b97f4a7c15ull;
+    for (u64 w : k) h = (h ^ w) * 0xbf58476d1ce4e5b9ull + (h >> 31);
+    return (size_t)h;
+  }
+};
+
+int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint, ppvm_sum** out) {
+  if (!out) return fail(PPVM_EINVAL, "null output pointer");
+  *out = nullptr;
+  if (n_qubits == 0) return fail(PPVM_EINVAL, "n_qubits must be positive");
+  u32 W = (n_qubits + 63) / 64;
+  u32 Wp = W <= 1 ? 1 : W <= 2 ? 2 : W <= 4 ? 4 : W <= 8 ? 8 : 0;
+  if (!Wp) return fail(PPVM_EINVAL, "n_qubits=%u exceeds the supported maximum of 512", n_qubits);
+  if (!(tau >= 0.0)) return fail(PPVM_EINVAL, "min_abs_coeff must be a non-negative number");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return fail(PPVM_ENODEV, "no HIP device available");
+  }
+  ppvm_sum* s = new ppvm_sum();
+  s->n = n_qubits;
+  s->W = Wp;
+  s->Wa = W;
+  s->tau = tau;
+  s->maxw = maxw;
+  s->trunc_active = tau > 0.0 || maxw != ~0ull;
+  s->clean = true;
+  HIPCHK(hipGetDevice(&s->device));
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return fail(PPVM_EHIP, "hipStreamCreate failed");
+  }
+  if (hipMalloc(&s->st, sizeof(DevState)) != hipSuccess ||
+      hipHostMalloc(&s->st_host, sizeof(DevState), hipHostMallocDefault) != hipSuccess ||
+      hipMalloc(&s->partials, kOverlapGrid * sizeof(double)) != hipSuccess) {
+    (void)hipGetLastError();
+    delete s;
+    return fail(PPVM_ENOMEM, "device allocation of handle state failed");
+  }
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, reinterpret_cast<u64*>(s->st),
+                     (u64)(sizeof(DevState) / 8), 0ull);
+  std::memset(s->st_host, 0, sizeof(DevState));
+  int rc = resize(s, pow2_at_least(std::max<u64>(kMinCap, cap_hint)), 0);
+  if (rc) {
+    free_buffers(s);
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return PPVM_OK;
+}
+
+}  // namespace
+
+// ================================ C ABI ===========================================
+extern "C" {
+
+const char* ppvm_last_error(void) { return g_err.c_str(); }
+
+int ppvm_device_info(char* buf, size_t cap) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    (void)hipGetLastError();
+    return fail(PPVM_ENODEV, "no HIP device available");
+  }
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  HIPCHK(hipGetDeviceProperties(&p, dev));
+  if (buf && cap) snprintf(buf, cap, "%s | %s | CUs=%d | mem=%.1f GiB", p.name, p.gcnArchName,
+                           p.multiProcessorCount, (double)p.totalGlobalMem / (1ull << 30));
+  return PPVM_OK;
+}
+
+int ppvm_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_pauli_weight, uint64_t capacity_hint,
+                ppvm_handle* out) {
+  return create_impl(n_qubits, min_abs_coeff, max_pauli_weight, capacity_hint, out);
+}
+
+int ppvm_destroy(ppvm_handle h) {
+  if (!h) return PPVM_OK;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_buffers(h);
+  if (h->st) (void)hipFree(h->st);
+  if (h->st_host) (void)hipHostFree(h->st_host);
+  if (h->partials) (void)hipFree(h->partials);
+  if (h->preserve) (void)hipFree(h->preserve);
+  for (auto e : h->ev) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete h;
+  return PPVM_OK;
+}
+
+int ppvm_words(ppvm_handle h, uint32_t* w) {
+  if (!h || !w) return fail(PPVM_EINVAL, "null argument");
+  *w = h->Wa;
+  return PPVM_OK;
+}
+
+static int upload_preserve(ppvm_handle h) {
+  if (h->preserve) (void)hipFree(h->preserve);
+  h->preserve = nullptr;
+  h->n_preserve = 0;
+  const size_t cnt = h->preserve_host.size() / (2 * h->W);
+  if (cnt) {
+    HIPCHK(hipMalloc(&h->preserve, h->preserve_host.size() * 8));
+    HIPCHK(hipMemcpy(h->preserve, h->preserve_host.data(), h->preserve_host.size() * 8, hipMemcpyHostToDevice));
+    h->n_preserve = (u32)cnt;
+  }
+  return PPVM_OK;
+}
+
+int ppvm_set_preserve(ppvm_handle h, const uint64_t* x, const uint64_t* z, size_t count) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  if (count && (!x || !z)) return fail(PPVM_EINVAL, "null key array");
+  int rc;
+  if ((rc = pump(h, true))) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const u32 W = h->W, Wa = h->Wa;
+  h->preserve_host.assign((size_t)count * 2 * W, 0);
+  for (size_t i = 0; i < count; ++i)
+    for (u32 p = 0; p < Wa; ++p) {
+      h->preserve_host[i * 2 * W + p] = x[i * Wa + p];
+      h->preserve_host[i * 2 * W + W + p] = z[i * Wa + p];
+    }
+  return upload_preserve(h);
+}
+
+int ppvm_clone(ppvm_handle h, ppvm_handle* out) {
+  if (!h || !out) return fail(PPVM_EINVAL, "null argument");
+  int rc;
+  if ((rc = pump(h, true))) return rc;
+  if ((rc = sync_check(h))) return rc;
+  ppvm_sum* c = nullptr;
+  if ((rc = create_impl(h->n, h->tau, h->maxw, h->cap, &c))) return rc;
+  const CountSlot cs = h->st_host->slot[h->op_seq & 1];
+  const u32 np = 2 * h->W;
+  if (cs.n_entries) {
+    hipLaunchKernelGGL(k_copy_arena, dim3(grid_for(cs.n_entries, 256, 4096)), dim3(256), 0, h->stream,
+                       h->planes[h->cur], h->coef[h->cur], h->cap, c->planes[c->cur], c->coef[c->cur], c->cap,
+                       np, cs.n_entries);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(&c->st->slot[c->op_seq & 1], &cs, sizeof(CountSlot), hipMemcpyHostToDevice));
+  c->n_bound = cs.n_entries;
+  c->clean = h->clean;
+  c->preserve_host = h->preserve_host;
+  if ((rc = upload_preserve(c))) {
+    ppvm_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return PPVM_OK;
+}
+
+int ppvm_add_terms(ppvm_handle h, const uint64_t* x, const uint64_t* z, const double* c, size_t count) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  if (count && (!x || !z || !c)) return fail(PPVM_EINVAL, "null term array");
+  if (count == 0) return PPVM_OK;
+  const u32 W = h->W, Wa = h->Wa, np = 2 * W;
+  // bits beyond n_qubits must be clear
+  const u32 nq = h->n;
+  for (size_t i = 0; i < count; ++i)
+    for (u32 p = 0; p < Wa; ++p) {
+      u64 valid = (p * 64 + 64 <= nq) ? ~0ull : (p * 64 >= nq ? 0ull : ((1ull << (nq - p * 64)) - 1));
+      if ((x[i * Wa + p] & ~valid) || (z[i * Wa + p] & ~valid))
+        return fail(PPVM_EINVAL, "term %zu has bits set beyond n_qubits=%u", i, nq);
+    }
+  int rc;
+  std::vector<u64> keys;
+  std::vector<double> cs;
+  if ((rc = download_live(h, keys, cs))) return rc;
+  // entry(k).and_modify(+=).or_insert(c), in order (map/hashmap.rs:40-44)
+  std::unordered_map<std::vector<u64>, size_t, KeyHash> idx(16, KeyHash{np});
+  std::vector<u64> k(np);
+  for (size_t i = 0; i < cs.size(); ++i) {
+    std::copy(keys.begin() + i * np, keys.begin() + (i + 1) * np, k.begin());
+    idx.emplace(k, i);
+  }
+  for (size_t i = 0; i < count; ++i) {
+    std::fill(k.begin(), k.end(), 0ull);
+    for (u32 p = 0; p < Wa; ++p) {
+      k[p] = x[i * Wa + p];
+      k[W + p] = z[i * Wa + p];
+    }
+    auto it = idx.find(k);
+    if (it != idx.end()) {
+      cs[it->second] += c[i];
+    } else {
+      idx.emplace(k, cs.size());
+      keys.insert(keys.end(), k.begin(), k.end());
+      cs.push_back(c[i]);
+    }
+  }
+  if ((rc = upload(h, keys, cs))) return rc;
+  h->clean = !h->trunc_active;
+  return PPVM_OK;
+}
+
+int ppvm_x(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_X, t, n, 0.0); }
+int ppvm_y(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_Y, t, n, 0.0); }
+int ppvm_z(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_Z, t, n, 0.0); }
+int ppvm_h(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_H, t, n, 0.0); }
+int ppvm_s(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_S, t, n, 0.0); }
+int ppvm_s_dag(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_S_DAG, t, n, 0.0); }
+int ppvm_sqrt_x(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_SQRT_X, t, n, 0.0); }
+int ppvm_sqrt_x_dag(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_SQRT_X_DAG, t, n, 0.0); }
+int ppvm_sqrt_y(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_SQRT_Y, t, n, 0.0); }
+int ppvm_sqrt_y_dag(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_SQRT_Y_DAG, t, n, 0.0); }
+int ppvm_cnot(ppvm_handle h, const uint32_t* p, size_t n) { return gate_2q(h, PPVM_OP_CNOT, p, n, 0.0); }
+int ppvm_cz(ppvm_handle h, const uint32_t* p, size_t n) { return gate_2q(h, PPVM_OP_CZ, p, n, 0.0); }
+int ppvm_cy(ppvm_handle h, const uint32_t* p, size_t n) { return gate_2q(h, PPVM_OP_CY, p, n, 0.0); }
+int ppvm_rx(ppvm_handle h, const uint32_t* t, size_t n, double th) { return gate_1q(h, PPVM_OP_RX, t, n, th); }
+int ppvm_ry(ppvm_handle h, const uint32_t* t, size_t n, double th) { return gate_1q(h, PPVM_OP_RY, t, n, th); }
+int ppvm_rz(ppvm_handle h, const uint32_t* t, size_t n, double th) { return gate_1q(h, PPVM_OP_RZ, t, n, th); }
+int ppvm_rxx(ppvm_handle h, const uint32_t* p, size_t n, double th) { return gate_2q(h, PPVM_OP_RXX, p, n, th); }
+int ppvm_ryy(ppvm_handle h, const uint32_t* p, size_t n, double th) { return gate_2q(h, PPVM_OP_RYY, p, n, th); }
+int ppvm_rzz(ppvm_handle h, const uint32_t* p, size_t n, double th) { return gate_2q(h, PPVM_OP_RZZ, p, n, th); }
+
+int ppvm_r(ppvm_handle h, uint32_t q, double axis_angle, double theta) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  int rc;
+  if ((rc = push_gate(h, PPVM_OP_RZ, q, 0, axis_angle, nullptr, 0))) return rc;
+  if ((rc = push_gate(h, PPVM_OP_RX, q, 0, theta, nullptr, 0))) return rc;
+  return push_gate(h, PPVM_OP_RZ, q, 0, -axis_angle, nullptr, 0);
+}
+
+int ppvm_pauli_error(ppvm_handle h, uint32_t q, const double p[3]) {
+  if (!h || !p) return fail(PPVM_EINVAL, "null argument");
+  return push_gate(h, PPVM_OP_PAULI_ERROR, q, 0, 0.0, p, 3);
+}
+int ppvm_depolarize1(ppvm_handle h, uint32_t q, double p) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  return push_gate(h, PPVM_OP_DEPOLARIZE1, q, 0, 0.0, &p, 1);
+}
+int ppvm_depolarize2(ppvm_handle h, uint32_t a, uint32_t b, double p) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  return push_gate(h, PPVM_OP_DEPOLARIZE2, a, b, 0.0, &p, 1);
+}
+int ppvm_two_qubit_pauli_error(ppvm_handle h, uint32_t a, uint32_t b, const double p[15]) {
+  if (!h || !p) return fail(PPVM_EINVAL, "null argument");
+  return push_gate(h, PPVM_OP_PAULI_ERROR2, a, b, 0.0, p, 15);
+}
+
+int ppvm_truncate(ppvm_handle h) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  ppvm_op op{};
+  op.code = PPVM_OP_TRUNCATE;
+  return enqueue(h, op);
+}
+
+int ppvm_apply(ppvm_handle h, const ppvm_op* ops, size_t n) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  if (n && !ops) return fail(PPVM_EINVAL, "null op array");
+  for (size_t i = 0; i < n; ++i) {
+    int rc = validate(h, ops[i]);
+    if (rc) return rc;
+  }
+  for (size_t i = 0; i < n; ++i) {
+    int rc = enqueue(h, ops[i]);
+    if (rc) return rc;
+  }
+  return PPVM_OK;
+}
+
+int ppvm_synchronize(ppvm_handle h) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  int rc;
+  if ((rc = pump(h, true))) return rc;
+  return sync_check(h);
+}
+
+int ppvm_len(ppvm_handle h, uint64_t* out) {
+  if (!h || !out) return fail(PPVM_EINVAL, "null argument");
+  int rc;
+  if ((rc = ppvm_synchronize(h))) return rc;
+  const CountSlot& c = h->st_host->slot[h->op_seq & 1];
+  *out = c.n_entries - c.n_dead;
+  return PPVM_OK;
+}
+
+int ppvm_overlap_with_zero(ppvm_handle h, double* out) {
+  if (!h || !out) return fail(PPVM_EINVAL, "null argument");
+  int rc;
+  if ((rc = ppvm_synchronize(h))) return rc;
+  if ((rc = overlap_w(h))) return rc;
+  if ((rc = read_state(h))) return rc;
+  *out = h->st_host->overlap;
+  return PPVM_OK;
+}
+
+int ppvm_overlap(ppvm_handle a, ppvm_handle b, double* out) {
+  if (!a || !b || !out) return fail(PPVM_EINVAL, "null argument");
+  if (a->n != b->n) return fail(PPVM_EINVAL, "overlap of sums over %u and %u qubits", a->n, b->n);
+  return fail(PPVM_EINVAL, "ppvm_overlap between two sums is not available in this build");
+}
+
+int ppvm_export_terms(ppvm_handle h, uint64_t* x, uint64_t* z, double* c, size_t cap, size_t* n) {
+  if (!h || !n) return fail(PPVM_EINVAL, "null argument");
+  std::vector<u64> keys;
+  std::vector<double> cs;
+  int rc;
+  if ((rc = download_live(h, keys, cs))) return rc;
+  *n = cs.size();
+  if (cap < cs.size()) return fail(PPVM_EINVAL, "export buffer holds %zu terms, %zu needed", cap, cs.size());
+  if (cs.size() && (!x || !z || !c)) return fail(PPVM_EINVAL, "null export array");
+  const u32 W = h->W, Wa = h->Wa, np = 2 * W;
+  for (size_t i = 0; i < cs.size(); ++i) {
+    for (u32 p = 0; p < Wa; ++p) {
+      x[i * Wa + p] = keys[i * np + p];
+      z[i * Wa + p] = keys[i * np + W + p];
+    }
+    c[i] = cs[i];
+  }
+  return PPVM_OK;
+}
+
+int ppvm_get_stats(ppvm_handle h, ppvm_stats* out) {
+  if (!h || !out) return fail(PPVM_EINVAL, "null argument");
+  int rc;
+  if ((rc = ppvm_synchronize(h))) return rc;
+  const DevState& d = *h->st_host;
+  const CountSlot& c = d.slot[h->op_seq & 1];
+  *out = h->hs;
+  out->term_updates = d.term_updates;
+  out->gates = d.gates;
+  out->rotations = d.rotations;
+  out->n_entries = c.n_entries;
+  out->n_live = c.n_entries - c.n_dead;
+  out->capacity = h->cap;
+  return PPVM_OK;
+}
+
+int ppvm_profile_enable(ppvm_handle h, int enable) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  int rc;
+  if ((rc = ppvm_synchronize(h))) return rc;
+  h->prof = enable != 0;
+  for (int i = 0; i < 8; ++i) {
+    h->prof_ms[i] = 0.0;
+    h->prof_n[i] = 0;
+  }
+  return PPVM_OK;
+}
+
+int ppvm_profile_read(ppvm_handle h, double ms[8], uint64_t launches[8]) {
+  if (!h || !ms || !launches) return fail(PPVM_EINVAL, "null argument");
+  int rc;
+  if ((rc = ppvm_synchronize(h))) return rc;
+  for (int i = 0; i < 8; ++i) {
+    ms[i] = h->prof_ms[i];
+    launches[i] = h->prof_n[i];
+  }
+  return PPVM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+"""Shared helpers for the parity tests (GPU engine vs CPU oracle)."""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle.pyoracle import OracleSum
+from ppvm_amd import ops as O
+
+
+def oracle_for(w) -> OracleSum:
+    s = OracleSum(w.n_qubits, w.min_abs_coeff, w.max_pauli_weight)
+    s.add_strings([t for t, _ in w.observable], [c for _, c in w.observable])
+    return s
+
+
+def gpu_for(w, capacity_hint: int = 0):
+    from ppvm_amd import PauliSum
+    return PauliSum.new(w.n_qubits, w.observable, min_abs_coeff=w.min_abs_coeff,
+                        max_pauli_weight=w.max_pauli_weight, capacity_hint=capacity_hint)
+
+
+def assert_same_terms(gpu_export, oracle_export, ordered: bool = True, ctx: str = ""):
+    """Bit-identical keys and coefficients (and, if ordered, identical order)."""
+    gx, gz, gc = gpu_export
+    ox, oz, oc = oracle_export
+    assert len(gc) == len(oc), f"{ctx}: {len(gc)} GPU terms vs {len(oc)} oracle terms"
+    if not ordered:
+        def keyed(x, z, c):
+            k = np.concatenate([x, z], axis=1)
+            order = np.lexsort(k.T[::-1])
+            return k[order], c[order]
+        gk, gc = keyed(gx, gz, gc)
+        ok, oc = keyed(ox, oz, oc)
+        assert np.array_equal(gk, ok), f"{ctx}: term sets differ"
+    else:
+        assert np.array_equal(gx, ox) and np.array_equal(gz, oz), f"{ctx}: keys/order differ"
+    gb = gc.view(np.uint64)
+    ob = oc.view(np.uint64)
+    bad = np.nonzero(gb != ob)[0]
+    assert len(bad) == 0, f"{ctx}: {len(bad)} coefficients differ, first {gc[bad[0]]!r} vs {oc[bad[0]]!r}"
+
+
+def run_segments(gpu, ref: OracleSum, program: np.ndarray, marks, check_terms=True, ordered=True,
+                 ctx=""):
+    """Apply program up to each mark on both sides and compare."""
+    prev = 0
+    for li, m in enumerate(marks):
+        seg = program[prev:m + 1]
+        gpu.apply(seg)
+        ref.apply(seg)
+        prev = m + 1
+        assert len(gpu) == len(ref), f"{ctx} layer {li}: len {len(gpu)} vs {len(ref)}"
+        og, oo = gpu.overlap_with_zero(), ref.overlap_with_zero()
+        assert abs(og - oo) <= 1e-10, f"{ctx} layer {li}: overlap {og!r} vs {oo!r}"
+        if check_terms:
+            assert_same_terms(gpu._export(), ref.export(), ordered, f"{ctx} layer {li}")
+    if prev < len(program):
+        gpu.apply(program[prev:])
+        ref.apply(program[prev:])
+
+
+def random_terms(rng, n, count):
+    """count distinct random Pauli strings on n qubits."""
+    seen = set()
+    out = []
+    while len(out) < count:
+        s = "".join("IXYZ"[v] for v in rng.integers(0, 4, size=n))
+        if s not in seen:
+            seen.add(s)
+            out.append(s)
+    return out
+
+
+def random_program(rng, n, ngates, trunc_every=True, kinds=None):
+    p = O.Program()
+    kinds = kinds or ["x", "y", "z", "h", "s", "s_dag", "sqrt_x", "sqrt_x_dag", "sqrt_y",
+                      "sqrt_y_dag", "cnot", "cz", "cy", "rx", "ry", "rz", "rxx", "ryy", "rzz"]
+    codes = {**O.CLIFFORD_1Q, **O.CLIFFORD_2Q, **O.ROTATION_1Q, **O.ROTATION_2Q}
+    for _ in range(ngates):
+        k = kinds[rng.integers(0, len(kinds))]
+        c = codes[k]
+        a = int(rng.integers(0, n))
+        b = int(rng.integers(0, n - 1))
+        if b >= a:
+            b += 1
+        th = float(rng.uniform(-np.pi, np.pi))
+        p.add(c, a, b if c in O.TWO_QUBIT else 0, theta=th)
+        if trunc_every:
+            p.truncate()
+    return p.array()

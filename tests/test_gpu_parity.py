@@ -1,0 +1,158 @@
+"""GPU engine vs CPU oracle: bit-identical term sets and coefficients.
+
+Every test drives the HIP library through the C ABI (via the ctypes shim) and
+replays the same op program on the oracle.  Bar: identical keys, identical
+coefficient bits and identical (insertion) order; overlap_with_zero within
+1e-10 absolute (the GPU sums in a different association).
+"""
+
+import numpy as np
+import pytest
+
+from ppvm_amd import ops as O
+from ppvm_amd import workloads as WL
+from tests.helpers import (assert_same_terms, gpu_for, oracle_for, random_program, random_terms,
+                           run_segments)
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(n, strings, coeffs, tau=0.0, maxw=None, cap=0):
+    from oracle.pyoracle import OracleSum
+    from ppvm_amd import PauliSum
+    g = PauliSum(strings, n, coefficients=coeffs, min_abs_coeff=tau, max_pauli_weight=maxw,
+                 capacity_hint=cap)
+    r = OracleSum(n, tau, maxw)
+    r.add_strings(strings, coeffs)
+    return g, r
+
+
+def test_device_present():
+    from ppvm_amd import _native
+    info = _native.device_info()
+    print(info)
+    assert "gfx950" in info
+
+
+@pytest.mark.parametrize("n", [5, 70, 130, 300])
+def test_every_gate_on_random_terms(n):
+    """Each gate kind on 200 random terms, no truncation (W = 1, 2, 4, 8)."""
+    rng = np.random.default_rng(1000 + n)
+    strings = random_terms(rng, n, 200)
+    coeffs = rng.normal(size=200)
+    for kind in ["x", "y", "z", "h", "s", "s_dag", "sqrt_x", "sqrt_x_dag", "sqrt_y", "sqrt_y_dag",
+                 "cnot", "cz", "cy", "rx", "ry", "rz", "rxx", "ryy", "rzz"]:
+        g, r = _pair(n, strings, coeffs)
+        prog = random_program(rng, n, 6, trunc_every=False, kinds=[kind])
+        g.apply(prog)
+        r.apply(prog)
+        assert_same_terms(g._export(), r.export(), True, f"n={n} gate={kind}")
+
+
+@pytest.mark.parametrize("n,tau,maxw", [(6, 1e-3, None), (40, 1e-2, 6), (100, 1e-4, 10)])
+def test_random_circuits_truncated(n, tau, maxw):
+    rng = np.random.default_rng(7 * n)
+    strings = random_terms(rng, n, 20)
+    coeffs = rng.normal(size=20)
+    g, r = _pair(n, strings, coeffs, tau, maxw)
+    prog = random_program(rng, n, 120, trunc_every=True)
+    marks = list(range(1, len(prog), 20))
+    run_segments(g, r, prog, marks, ctx=f"random n={n}")
+
+
+def test_chain_tfim_every_layer():
+    w = WL.chain_tfim()
+    g, r = gpu_for(w), oracle_for(w)
+    run_segments(g, r, w.program, w.layer_marks, ctx="chain_tfim")
+    assert g.stats()["term_updates"] == r.term_updates()
+
+
+def test_lattice_prefix_every_layer():
+    w = WL.truncate_steps(WL.lattice_tfim(), 5)
+    g, r = gpu_for(w), oracle_for(w)
+    run_segments(g, r, w.program, w.layer_marks, ctx="lattice_tfim")
+    assert g.stats()["term_updates"] == r.term_updates()
+
+
+def test_random_circuit_every_layer():
+    w = WL.random_circuit()
+    g, r = gpu_for(w), oracle_for(w)
+    run_segments(g, r, w.program, w.layer_marks, ctx="random_circuit")
+
+
+def test_growth_from_tiny_capacity():
+    """Arena overflow is detected on the device and replayed after growth."""
+    w = WL.truncate_steps(WL.lattice_tfim(), 4)
+    g, r = gpu_for(w, capacity_hint=1), oracle_for(w)
+    g.apply(w.program)
+    r.apply(w.program)
+    assert_same_terms(g._export(), r.export(), True, "growth")
+    st = g.stats()
+    assert st["regrows"] >= 1
+
+
+def test_deferred_truncation_cadence():
+    """Per-call cadence: truncate only after each layer (Python *_many calls)."""
+    w = WL.chain_tfim(n=16, steps=4)
+    prog = w.program[w.program["code"] != O.TRUNCATE]
+    p2 = O.Program()
+    for i, op in enumerate(prog):
+        p2.add(int(op["code"]), int(op["q0"]), int(op["q1"]), float(op["theta"]))
+        if i % 16 == 15:
+            p2.truncate()
+    arr = p2.array()
+    g, r = gpu_for(w), oracle_for(w)
+    g.apply(arr)
+    r.apply(arr)
+    assert_same_terms(g._export(), r.export(), True, "deferred")
+
+
+def test_noise_channels():
+    rng = np.random.default_rng(5)
+    n = 9
+    strings = random_terms(rng, n, 60)
+    coeffs = rng.normal(size=60)
+    g, r = _pair(n, strings, coeffs, 1e-3)
+    p = O.Program()
+    for _ in range(20):
+        a, b = rng.choice(n, size=2, replace=False)
+        p.add(O.PAULI_ERROR, int(a), p=rng.uniform(0, 0.05, 3)).truncate()
+        p.add(O.DEPOLARIZE1, int(b), p=[0.03]).truncate()
+        p.add(O.DEPOLARIZE2, int(a), int(b), p=[0.02]).truncate()
+        p.add(O.PAULI_ERROR2, int(b), int(a), p=rng.uniform(0, 0.01, 15)).truncate()
+        p.add(O.RX, int(a), theta=0.3).truncate()
+    g.apply(p.array())
+    r.apply(p.array())
+    assert_same_terms(g._export(), r.export(), True, "noise")
+
+
+def test_preserve_strings_kept():
+    n = 4
+    g, r = _pair(n, ["ZIII", "XIII"], [1.0, 1e-9], tau=1e-6)
+    g2 = type(g)(["ZIII", "XIII"], n, coefficients=[1.0, 1e-9], min_abs_coeff=1e-6,
+                 preserve_strings=["XIII"])
+    g2.truncate()
+    assert dict(g2.terms) == {"ZIII": 1.0, "XIII": 1e-9}
+    g.truncate()
+    assert dict(g.terms) == {"ZIII": 1.0}
+
+
+def test_clone_is_deep():
+    w = WL.truncate_steps(WL.chain_tfim(), 2)
+    g = gpu_for(w)
+    g.apply(w.program)
+    c = g.copy()
+    g.rx(3, 0.7)
+    r = oracle_for(w)
+    r.apply(w.program)
+    assert_same_terms(c._export(), r.export(), True, "clone")
+
+
+def test_determinism():
+    w = WL.truncate_steps(WL.lattice_tfim(), 4)
+    a, b = gpu_for(w), gpu_for(w)
+    a.apply(w.program)
+    b.apply(w.program)
+    ea, eb = a._export(), b._export()
+    assert all(np.array_equal(u, v) for u, v in zip(ea, eb))
+    assert a.overlap_with_zero() == b.overlap_with_zero()
