@@ -102,7 +102,9 @@ class PauliSum:
                     f"{n_qubits}, but got term of length {len(term)}: {term!r}")
         if max_loss_weight is not None:
             raise NotImplementedError("lossy Pauli words are not supported by this engine")
-        coefficients = tuple(coefficients) if coefficients else (1.0,) * len(initial_terms)
+        coefficients = tuple(float(c) for c in coefficients)
+        if not coefficients:
+            coefficients = (1.0,) * len(initial_terms)
         if len(coefficients) != len(initial_terms):
             raise ValueError("Initial terms and coefficients need to be of same length!")
         self.initial_terms = tuple(initial_terms)
