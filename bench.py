@@ -1,0 +1,242 @@
+"""Headline benchmark: Pauli-term updates/s of the 10x10 lattice TFIM on one MI355X.
+
+One "step" is one full backward propagation of the lattice workload (10x10
+open lattice, h=3.04, J=1, dt=0.05, 20 Trotter steps = 5,600 rotation gates,
+threshold 1e-5, max weight 10, truncation after every gate) from the
+observable Z_(5,5).  Inputs are resident in HBM before the timed region: each
+timed step runs on a handle created (and sized for the peak term count) before
+the clock starts.  A term update is one live input term processed by one gate
+(U = sum over gates of live input terms; identical on CPU and GPU because the
+term sets are identical).
+
+Prints one JSON line.  Extra objects:
+  roofline     dominant kernel's algorithmic bytes / its average launch time
+               (HIP events on the handle's stream, profiled pass);
+  cpu_baseline the CPU oracle (single thread) on a bounded prefix of the same
+               workload, timed on this host in the same run;
+  parity       the GPU term set on that prefix vs the oracle's (bit-identical).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Pauli-term updates/s, 10x10 TFIM vs CPU; surrogate sweep speedup @ |err|<=1e-3"
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _make(w, cap: int = 0):
+    from ppvm_amd import PauliSum
+    return PauliSum.new(w.n_qubits, w.observable, min_abs_coeff=w.min_abs_coeff,
+                        max_pauli_weight=w.max_pauli_weight, capacity_hint=cap)
+
+
+def phase_bytes(cnt: dict, W: int) -> dict:
+    """Algorithmic HBM bytes per kernel kind, summed over the run.
+
+    k_rot_scan : 8 B per test word of every scanned entry, the rest of the key
+                 and the coefficient of every anticommuting term
+                 (16W + 8 - 8 w_t, w_t ~ test words), and one worklist record
+                 (16W + 16 B) per anticommuting term.
+    k_rot_pair : one worklist record read per anticommuting term, one
+                 coefficient written per merged term.
+    k_rot_emit : one worklist record read per anticommuting term, one
+                 coefficient written per unpaired parent, one term
+                 (16W + 8 B) written per appended child.
+    """
+    S = 16 * W + 8
+    rec = 16 * W + 16
+    na = cnt["wl_total"]
+    npair = cnt["pairs_total"]
+    nch = cnt["children_total"]
+    scan_words = cnt["scan_words"]
+    wt_avg = scan_words / max(1, cnt["scanned"])
+    scan = 8 * scan_words + na * (S - 8 * wt_avg) + na * rec
+    pair = na * rec + 2 * npair * 8
+    emit = na * rec + (na - 2 * npair) * 8 + nch * S
+    return {"scan": scan, "pair": pair, "emit": emit}
+
+
+def run(args) -> dict:
+    from ppvm_amd import _native
+    from ppvm_amd import workloads as WL
+
+    info = _native.device_info()
+    _log(f"device: {info}")
+    w = WL.lattice_tfim()
+    W = 2
+    # warm-up: learn the peak size and term-update count, load the kernels
+    t0 = time.perf_counter()
+    g = _make(w)
+    g.apply(w.program)
+    g.synchronize()
+    warm_s = time.perf_counter() - t0
+    c0 = g._h.counters()
+    U = c0["term_updates"]
+    peak = c0["peak_live"]
+    cap = int(c0["capacity"])
+    ovl = g.overlap_with_zero()
+    n_final = len(g)
+    _log(f"warm-up run: {warm_s:.2f}s U={U} peak_live={peak} final={n_final} cap={cap} overlap={ovl:.12f}")
+    del g
+    for _ in range(max(0, args.warmup - 1)):
+        g = _make(w, cap)
+        g.apply(w.program)
+        g.synchronize()
+        del g
+
+    # timed region: K full propagations, each on a pre-sized handle
+    handles = [_make(w, cap) for _ in range(args.steps)]
+    for h in handles:
+        h.synchronize()
+    t0 = time.perf_counter()
+    for h in handles:
+        h.apply(w.program)
+        h.synchronize()
+    elapsed = time.perf_counter() - t0
+    for h in handles:
+        c = h._h.counters()
+        assert c["term_updates"] == U, "non-deterministic term-update count"
+        assert abs(h.overlap_with_zero() - ovl) <= 1e-10
+    del handles
+    value = U * args.steps / elapsed
+    _log(f"timed: {args.steps} x {elapsed / args.steps:.3f}s  -> {value:.4e} term-updates/s")
+
+    # profiled pass: per-kernel HIP-event timing on the handle's stream
+    g = _make(w, cap)
+    g._h.profile(True)
+    g.apply(w.program)
+    g.synchronize()
+    ms, launches = g._h.profile_read()
+    cnt = g._h.counters()
+    del g
+    names = ["k_rot_scan", "k_rot_pair", "k_rot_emit", "k_map", "k_overlap", "k_compact"]
+    pb = phase_bytes(cnt, W)
+    kern = {}
+    for i, nm in enumerate(names):
+        if launches[i]:
+            kern[nm] = {"ms_total": float(ms[i]), "launches": int(launches[i]),
+                        "avg_us": float(ms[i]) * 1e3 / int(launches[i])}
+    for key, nm in (("scan", "k_rot_scan"), ("pair", "k_rot_pair"), ("emit", "k_rot_emit")):
+        if nm in kern:
+            kern[nm]["alg_bytes_total"] = float(pb[key])
+            kern[nm]["achieved_GBs"] = pb[key] / (kern[nm]["ms_total"] * 1e-3) / 1e9
+    dom = max(kern, key=lambda k: kern[k]["ms_total"])
+    _log("kernels: " + json.dumps(kern))
+    dk = kern[dom]
+    achieved = dk.get("achieved_GBs", 0.0)
+    roofline = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "avg_launch_us": round(dk["avg_us"], 3),
+        "alg_bytes_per_launch": round(dk.get("alg_bytes_total", 0.0) / dk["launches"], 1),
+    }
+
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "term-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "lattice_tfim: 10x10 open TFIM, J=1, h=3.04, dt=0.05, 20 Trotter steps, "
+                        "Z(5,5), threshold 1e-5, max weight 10, truncate after every gate",
+            "gates": int(w.n_gates),
+            "term_updates_per_step": int(U),
+            "peak_live_terms": int(peak),
+            "final_terms": int(n_final),
+            "overlap_with_zero": ovl,
+            "device": info,
+        },
+        "roofline": roofline,
+        "kernels": kern,
+    }
+
+    # CPU baseline + parity on a bounded prefix of the same workload
+    if not args.no_cpu:
+        from oracle.pyoracle import OracleSum
+        ps = args.cpu_prefix_steps
+        p = WL.truncate_steps(w, ps)
+        ref = OracleSum(p.n_qubits, p.min_abs_coeff, p.max_pauli_weight)
+        ref.add_strings([s for s, _ in p.observable], [c for _, c in p.observable])
+        t0 = time.perf_counter()
+        ref.apply(p.program)
+        cpu_s = time.perf_counter() - t0
+        cpu_rate = ref.term_updates() / cpu_s
+        gp = _make(p, cap)
+        gp.synchronize()
+        t0 = time.perf_counter()
+        gp.apply(p.program)
+        gp.synchronize()
+        gpu_s = time.perf_counter() - t0
+        gx, gz, gc = gp._export()
+        ox, oz, oc = ref.export()
+        same = (len(gc) == len(oc) and np.array_equal(gx, ox) and np.array_equal(gz, oz)
+                and np.array_equal(gc.view(np.uint64), oc.view(np.uint64)))
+        gu = gp._h.counters()["term_updates"]
+        result["cpu_baseline"] = {
+            "value": cpu_rate,
+            "unit": "term-updates/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"first {ps} of 20 Trotter steps of the same lattice workload "
+                      f"({len(p.program)} ops, U={ref.term_updates()}, {cpu_s:.1f}s single-thread)",
+            "gpu_same_prefix": gu / gpu_s,
+        }
+        result["parity"] = {
+            "prefix_steps": ps,
+            "terms": int(len(oc)),
+            "bit_identical": bool(same),
+            "term_updates_equal": bool(gu == ref.term_updates()),
+            "overlap_abs_diff": abs(gp.overlap_with_zero() - ref.overlap_with_zero()),
+        }
+        result["speedup_vs_cpu"] = value / cpu_rate
+        _log(f"cpu: {cpu_rate:.4e} term-updates/s over {ps} steps ({cpu_s:.1f}s); parity={same}")
+    return result
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-prefix-steps", type=int, default=11)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of "
+                                   "this build (--gpus must be 1)", "n_gpus": args.gpus}))
+        return 2
+    res = run(args)
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

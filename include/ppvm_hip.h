@@ -191,6 +191,12 @@ int ppvm_apply(ppvm_handle h, const ppvm_op* ops, size_t n);
 /* Wait for all queued work; check device-side overflow and regrow if needed. */
 int ppvm_synchronize(ppvm_handle h);
 int ppvm_get_stats(ppvm_handle h, ppvm_stats* out);
+/* Device counters for the measurement harness, in this order: term_updates,
+ * gates, rotations, scanned entries, anticommuting (worklist) terms, merged
+ * pairs, appended children, truncated terms, map-program entries, map
+ * launches, scan plane-words, capacity, syncs, regrows, compactions,
+ * peak live terms. */
+int ppvm_get_counters(ppvm_handle h, uint64_t out[16]);
 
 /* Per-phase kernel timing (HIP events on the handle's stream), for the
  * measurement harness.  Phases: 0 rotation scan, 1 rotation pair/merge,

@@ -75,6 +75,7 @@ def _declare(L):
     d("ppvm_apply", _H, ctypes.c_void_p, ctypes.c_size_t)
     d("ppvm_synchronize", _H)
     d("ppvm_get_stats", _H, ctypes.POINTER(PpvmStats))
+    d("ppvm_get_counters", _H, _u64p)
     d("ppvm_profile_enable", _H, ctypes.c_int)
     d("ppvm_profile_read", _H, _f64p, _u64p)
 
@@ -214,6 +215,15 @@ class Handle:
         s = PpvmStats()
         check(lib().ppvm_get_stats(self.h, ctypes.byref(s)))
         return {name: int(getattr(s, name)) for name, _ in PpvmStats._fields_}
+
+    COUNTERS = ("term_updates", "gates", "rotations", "scanned", "wl_total", "pairs_total",
+                "children_total", "kills_total", "map_entries", "map_launches", "scan_words",
+                "capacity", "syncs", "regrows", "compactions", "peak_live")
+
+    def counters(self) -> dict:
+        out = np.zeros(16, dtype=np.uint64)
+        check(lib().ppvm_get_counters(self.h, u64p(out)))
+        return {k: int(v) for k, v in zip(self.COUNTERS, out)}
 
     def profile(self, enable: bool):
         check(lib().ppvm_profile_enable(self.h, 1 if enable else 0))

@@ -57,10 +57,11 @@ struct alignas(64) DevState {
   u64 kills_total;    // terms truncated
   u64 map_entries;    // entries visited by map programs
   u64 map_launches;
+  u64 scan_words;     // plane words read by rotation scans (entries x test words)
   double overlap;     // last overlap_with_zero result
   u32 ticket[4];      // self-resetting tile tickets (one per tiled kernel kind)
   u32 done_count[8];  // self-resetting block counters (last-block finalisers)
-  u64 pad[2];
+  u64 pad[1];
 };
 
 enum TicketKind { TK_SCAN = 0, TK_EMIT = 1, TK_COMPACT = 2 };
@@ -72,6 +73,7 @@ struct RotSpec {
   u32 two;                 // 1 for a two-qubit generator
   u32 gax, gaz, gbx, gbz;  // generator Pauli bits on a and b
   u32 trunc;               // fused truncation after the gate
+  u32 test_words;          // distinct plane words the scan reads per entry
   double sn, cs;           // sin(theta), cos(theta) from the host libm
 };
 

@@ -466,6 +466,13 @@ RotSpec make_rot(const ppvm_op& op, bool trunc) {
   }
   if (!g.two) g.b = g.a;
   g.trunc = trunc ? 1u : 0u;
+  {
+    // distinct plane words the scan reads: x words if the generator has a Z
+    // part, z words if it has an X part, for each distinct word index
+    const u32 nx = (g.gaz | g.gbz) ? 1u : 0u, nz = (g.gax | g.gbx) ? 1u : 0u;
+    const u32 nw = (g.two && (g.a >> 6) != (g.b >> 6)) ? 2u : 1u;
+    g.test_words = (nx + nz) * nw;
+  }
   // Float::sin_cos of the reference (crates/ppvm-traits/src/traits/coefficient.rs:55-57):
   // the host libm, once per gate.
   g.sn = std::sin(op.theta);
@@ -1073,6 +1080,19 @@ int ppvm_get_stats(ppvm_handle h, ppvm_stats* out) {
   out->n_entries = c.n_entries;
   out->n_live = c.n_entries - c.n_dead;
   out->capacity = h->cap;
+  return PPVM_OK;
+}
+
+int ppvm_get_counters(ppvm_handle h, uint64_t out[16]) {
+  if (!h || !out) return fail(PPVM_EINVAL, "null argument");
+  int rc;
+  if ((rc = ppvm_synchronize(h))) return rc;
+  const DevState& d = *h->st_host;
+  const u64 v[16] = {d.term_updates, d.gates,          d.rotations,   d.scanned,
+                     d.wl_total,     d.pairs_total,    d.children_total, d.kills_total,
+                     d.map_entries,  d.map_launches,   d.scan_words,  h->cap,
+                     h->hs.syncs,    h->hs.regrows,    h->hs.compactions, h->hs.peak_live};
+  for (int i = 0; i < 16; ++i) out[i] = v[i];
   return PPVM_OK;
 }
 

@@ -354,6 +354,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_emit(u64* __restrict__ planes,
     st->gates += 1;
     st->rotations += 1;
     st->scanned += n_entries;
+    st->scan_words += n_entries * g.test_words;
     st->wl_total += n_wl;
     st->children_total += appended;
     st->kills_total += kills;
