@@ -45,30 +45,35 @@ def _make(w, cap: int = 0):
                         max_pauli_weight=w.max_pauli_weight, capacity_hint=cap)
 
 
-def phase_bytes(cnt: dict, W: int) -> dict:
-    """Algorithmic HBM bytes per kernel kind, summed over the run.
+KERNELS = ["k_rot_count", "k_rot_gather", "k_rot_pair", "k_emit_count", "k_emit_write", "k_map",
+           "k_finish", "other"]
 
-    k_rot_scan : 8 B per test word of every scanned entry, the rest of the key
-                 and the coefficient of every anticommuting term
-                 (16W + 8 - 8 w_t, w_t ~ test words), and one worklist record
-                 (16W + 16 B) per anticommuting term.
-    k_rot_pair : one worklist record read per anticommuting term, one
-                 coefficient written per merged term.
-    k_rot_emit : one worklist record read per anticommuting term, one
-                 coefficient written per unpaired parent, one term
-                 (16W + 8 B) written per appended child.
+
+def phase_bytes(cnt: dict, W: int) -> dict:
+    """Algorithmic HBM bytes per kernel, summed over the run (S = 16W + 8 B per
+    term, R = 16W + 16 B per worklist record, N = scanned entries, N_a =
+    anticommuting terms, N_p = merged pairs, N_c = appended children):
+
+    k_rot_count  : 8 B per test word of every entry + 8 B coefficient of each
+                   anticommuting candidate + N/8 B of ballot masks.
+    k_rot_gather : N/8 B masks + N_a (S + R): key and coefficient read, record written.
+    k_rot_pair   : N_a R records read + 2 N_p 8 B coefficients written.
+    k_emit_count : N_a R records read + (N_a - 2 N_p) 8 B coefficients written + N_a/8 masks.
+    k_emit_write : N_a/8 masks + N_c (R + S): record read, child term written.
     """
     S = 16 * W + 8
-    rec = 16 * W + 16
+    R = 16 * W + 16
+    N = cnt["scanned"]
     na = cnt["wl_total"]
     npair = cnt["pairs_total"]
     nch = cnt["children_total"]
-    scan_words = cnt["scan_words"]
-    wt_avg = scan_words / max(1, cnt["scanned"])
-    scan = 8 * scan_words + na * (S - 8 * wt_avg) + na * rec
-    pair = na * rec + 2 * npair * 8
-    emit = na * rec + (na - 2 * npair) * 8 + nch * S
-    return {"scan": scan, "pair": pair, "emit": emit}
+    return {
+        "k_rot_count": 8 * cnt["scan_words"] + 8 * na + N / 8,
+        "k_rot_gather": N / 8 + na * (S + R),
+        "k_rot_pair": na * R + 2 * npair * 8,
+        "k_emit_count": na * R + (na - 2 * npair) * 8 + na / 8,
+        "k_emit_write": na / 8 + nch * (R + S),
+    }
 
 
 def run(args) -> dict:
@@ -124,17 +129,15 @@ def run(args) -> dict:
     ms, launches = g._h.profile_read()
     cnt = g._h.counters()
     del g
-    names = ["k_rot_scan", "k_rot_pair", "k_rot_emit", "k_map", "k_overlap", "k_compact"]
     pb = phase_bytes(cnt, W)
     kern = {}
-    for i, nm in enumerate(names):
+    for i, nm in enumerate(KERNELS):
         if launches[i]:
             kern[nm] = {"ms_total": float(ms[i]), "launches": int(launches[i]),
                         "avg_us": float(ms[i]) * 1e3 / int(launches[i])}
-    for key, nm in (("scan", "k_rot_scan"), ("pair", "k_rot_pair"), ("emit", "k_rot_emit")):
-        if nm in kern:
-            kern[nm]["alg_bytes_total"] = float(pb[key])
-            kern[nm]["achieved_GBs"] = pb[key] / (kern[nm]["ms_total"] * 1e-3) / 1e9
+            if nm in pb:
+                kern[nm]["alg_bytes_total"] = float(pb[nm])
+                kern[nm]["achieved_GBs"] = pb[nm] / (kern[nm]["ms_total"] * 1e-3) / 1e9
     dom = max(kern, key=lambda k: kern[k]["ms_total"])
     _log("kernels: " + json.dumps(kern))
     dk = kern[dom]
@@ -226,7 +229,7 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-prefix-steps", type=int, default=11)
+    ap.add_argument("--cpu-prefix-steps", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -8,7 +8,15 @@
 //   wl     : WLRec<W>[cap]  per-gate worklist of anticommuting terms (AoS, one
 //                           record = full key + coefficient + arena index).
 //   table  : u64[tcap]      open-addressed pair table (generation-tagged slots).
-//   status : u64[tiles]     decoupled look-back tile status words.
+//   masks  : u64[cap/64]    per-pass ballot bitmasks (bit j of word w = item 64w+j)
+//   parts  : u64[6][8192]   per-chunk counts / offsets and per-block partial sums.
+//
+// Stream compaction (worklist, children, dead-entry removal) is done without
+// atomics or inter-block waiting: a pass over n items is split into G
+// contiguous wave-sized chunks; a count kernel writes each chunk's ballot
+// masks and count, a single-block kernel turns counts into offsets, and a
+// write kernel re-reads only the masks to place every selected item.  The
+// output order is the input order, so results are deterministic.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -19,10 +27,10 @@ typedef uint32_t u32;
 
 namespace ppvm {
 
-constexpr int kBlock = 256;        // threads per block (4 wave64)
-constexpr int kItems = 4;          // entries per thread in tiled kernels
-constexpr int kTile = kBlock * kItems;
+constexpr int kBlock = 256;        // threads per block (4 wave64) for grid-stride kernels
 constexpr int kWaves = kBlock / 64;
+constexpr int kMaxParts = 8192;    // max wave-chunks / blocks whose partials are reduced
+constexpr int kScanThreads = 1024; // single-block scan / reduction kernels
 
 // Dead-entry marker: a quiet NaN with a fixed payload (sign ignored).
 constexpr u64 DEAD_BITS = 0x7ff8dead0000dea1ull;
@@ -45,8 +53,6 @@ struct alignas(64) DevState {
   CountSlot slot[2];
   u64 n_wl;           // worklist length of the current rotation
   u64 overflow;       // 0, or (op sequence number + 1) of the first overflow
-  u64 op_kills;       // kills of the op in flight (reset by its finaliser)
-  u64 op_updates;     // term updates of the map program in flight
   u64 term_updates;   // sum over ops of live input terms
   u64 gates;          // ops applied (rotations + map ops)
   u64 rotations;
@@ -59,13 +65,8 @@ struct alignas(64) DevState {
   u64 map_launches;
   u64 scan_words;     // plane words read by rotation scans (entries x test words)
   double overlap;     // last overlap_with_zero result
-  u32 ticket[4];      // self-resetting tile tickets (one per tiled kernel kind)
-  u32 done_count[8];  // self-resetting block counters (last-block finalisers)
-  u64 pad[1];
+  u64 pad[7];
 };
-
-enum TicketKind { TK_SCAN = 0, TK_EMIT = 1, TK_COMPACT = 2 };
-enum DoneKind { DK_SCAN = 0, DK_PAIR = 1, DK_EMIT = 2, DK_MAP = 3, DK_OVERLAP = 4, DK_COMPACT = 5 };
 
 // Rotation gate exp(-i theta/2 G) with G acting on qubits a (and b).
 struct RotSpec {
@@ -177,106 +178,6 @@ __device__ __forceinline__ u32 add_u32(u32* p, u32 v) {
 }
 __device__ __forceinline__ void st_u32(u32* p, u32 v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---- tiles: in-order tickets + decoupled look-back -------------------------
-// Status word: gen (24 bits) | flag (2 bits) | value (38 bits).
-constexpr u64 ST_AGG = 1, ST_INCL = 2;
-__device__ __forceinline__ u64 st_pack(u32 gen, u64 flag, u64 v) {
-  return ((u64)gen << 40) | (flag << 38) | (v & ((1ull << 38) - 1));
-}
-
-// Takes the next tile index; tiles are handed out in increasing order so a
-// tile only ever waits on tiles that are already running.  Returns -1 when all
-// tiles are taken; the block taking the very last ticket resets the counter
-// for the next launch.
-__device__ __forceinline__ long long next_tile(u32* ticket, u64 n_tiles, u32* lds_slot) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    u32 t = add_u32(ticket, 1u);
-    if ((u64)t >= n_tiles) {
-      if ((u64)t == n_tiles + gridDim.x - 1) st_u32(ticket, 0u);
-      *lds_slot = 0xffffffffu;
-    } else {
-      *lds_slot = t;
-    }
-  }
-  __syncthreads();
-  u32 v = *lds_slot;
-  return v == 0xffffffffu ? -1 : (long long)v;
-}
-
-// Exclusive prefix of this tile over all earlier tiles (thread 0 computes,
-// result returned to every thread through lds).
-__device__ __forceinline__ u64 lookback(u64* status, u32 gen, u64 tile, u64 agg, u64* lds_u64) {
-  if (threadIdx.x == 0) {
-    u64 prefix = 0;
-    if (tile == 0) {
-      st_release(&status[0], st_pack(gen, ST_INCL, agg));
-    } else {
-      st_release(&status[tile], st_pack(gen, ST_AGG, agg));
-      long long j = (long long)tile - 1;
-      while (true) {
-        u64 s = ld_acquire(&status[j]);
-        u64 flag = (s >> 38) & 3ull;
-        if ((u32)(s >> 40) != gen || flag == 0) continue;  // predecessor not yet published
-        prefix += s & ((1ull << 38) - 1);
-        if (flag == ST_INCL) break;
-        --j;
-      }
-      st_release(&status[tile], st_pack(gen, ST_INCL, prefix + agg));
-    }
-    *lds_u64 = prefix;
-  }
-  __syncthreads();
-  return *lds_u64;
-}
-
-// Block-wide ranks of 0/1 flags over kItems rows: element (row i, thread t)
-// precedes (row i', thread t') iff (i, t) < (i', t').  pos[i] is the rank of
-// this thread's element in row i among flagged elements; returns the total.
-__device__ __forceinline__ u32 tile_ranks(const bool (&f)[kItems], u32 (&pos)[kItems], u32* lds_cnt) {
-  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  u32 inwave[kItems];
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    u64 b = __ballot(f[i]);
-    inwave[i] = (u32)__popcll(b & ((1ull << lane) - 1ull));
-    if (lane == 0) lds_cnt[i * kWaves + wave] = (u32)__popcll(b);
-  }
-  __syncthreads();
-  u32 run = 0;
-  u32 base[kItems];
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-#pragma unroll
-    for (int w = 0; w < kWaves; ++w) {
-      u32 v = lds_cnt[i * kWaves + w];
-      if ((u32)w == wave) base[i] = run;
-      run += v;
-    }
-  }
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) pos[i] = base[i] + inwave[i];
-  return run;
-}
-
-// Last-block detection: every block calls this once after its work; exactly
-// one block (the last to arrive) gets true and must reset nothing itself —
-// the counter is reset here for the next launch.
-__device__ __forceinline__ bool last_block(u32* done, u32* lds_flag) {
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    u32 d = add_u32(done, 1u);
-    bool last = d == gridDim.x - 1;
-    if (last) st_u32(done, 0u);
-    *lds_flag = last ? 1u : 0u;
-  }
-  __syncthreads();
-  bool r = *lds_flag != 0u;
-  if (r) __threadfence();
-  return r;
 }
 
 // Deterministic block sum of a u64 (for counters).

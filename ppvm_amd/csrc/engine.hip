@@ -111,8 +111,8 @@ struct ppvm_sum {
   void* wl = nullptr;
   u64* table = nullptr;
   u64 tcap = 0;
-  u64* status = nullptr;
-  u64 n_status = 0;
+  u64* masks = nullptr;     // ballot masks, cap/64 + kMaxParts words
+  u64* parts = nullptr;     // 6 x kMaxParts per-chunk counts / partial sums
   double* partials = nullptr;
   DevState* st = nullptr;
   DevState* st_host = nullptr;  // pinned
@@ -120,7 +120,6 @@ struct ppvm_sum {
   u32 n_preserve = 0;
   std::vector<u64> preserve_host;
   u64 op_seq = 0;
-  u32 lb_gen = 0;
   u32 tab_gen = 0;
   bool clean = true;
   bool broken = false;
@@ -136,6 +135,7 @@ struct ppvm_sum {
   size_t ev_used = 0;
   double prof_ms[8] = {0};
   u64 prof_n[8] = {0};
+  u64* part(int i) const { return parts + (size_t)i * kMaxParts; }
 
   size_t wl_rec() const { return 16 * (size_t)W + 16; }
   TruncSpec trunc_spec() const {
@@ -189,14 +189,6 @@ int grid_for(u64 n, u64 per_block, int maxg) {
   return (int)g;
 }
 
-u32 next_lb_gen(ppvm_sum* s, int* rc) {
-  *rc = PPVM_OK;
-  if (s->lb_gen >= kGenLimit) {
-    hipLaunchKernelGGL(k_fill_u64, dim3(256), dim3(256), 0, s->stream, s->status, s->n_status, 0ull);
-    s->lb_gen = 0;
-  }
-  return ++s->lb_gen;
-}
 u32 next_tab_gen(ppvm_sum* s) {
   if (s->tab_gen >= kGenLimit) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1024), dim3(256), 0, s->stream, s->table, s->tcap, 0ull);
@@ -215,10 +207,10 @@ void free_buffers(ppvm_sum* s) {
   }
   if (s->wl) (void)hipFree(s->wl);
   if (s->table) (void)hipFree(s->table);
-  if (s->status) (void)hipFree(s->status);
+  if (s->masks) (void)hipFree(s->masks);
   s->wl = nullptr;
   s->table = nullptr;
-  s->status = nullptr;
+  s->masks = nullptr;
 }
 
 // (Re)allocate all capacity-dependent buffers for capacity new_cap, keeping the
@@ -257,25 +249,21 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
   s->cur = 0;
   if (s->wl) (void)hipFree(s->wl);
   if (s->table) (void)hipFree(s->table);
-  if (s->status) (void)hipFree(s->status);
+  if (s->masks) (void)hipFree(s->masks);
   s->wl = nullptr;
   s->table = nullptr;
-  s->status = nullptr;
+  s->masks = nullptr;
   s->tcap = pow2_at_least(2 * new_cap);
-  s->n_status = new_cap / kTile + 2;
   if (hipMalloc(&s->wl, new_cap * s->wl_rec()) != hipSuccess ||
       hipMalloc(&s->table, s->tcap * 8) != hipSuccess ||
-      hipMalloc(&s->status, s->n_status * 8) != hipSuccess) {
+      hipMalloc(&s->masks, (new_cap / 64 + kMaxParts + 64) * 8) != hipSuccess) {
     (void)hipGetLastError();
     return fail(PPVM_ENOMEM, "device allocation of work buffers for %llu terms failed",
                 (unsigned long long)new_cap);
   }
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(s->tcap, 256, 4096)), dim3(256), 0, s->stream, s->table,
                      s->tcap, 0ull);
-  hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(s->n_status, 256, 1024)), dim3(256), 0, s->stream,
-                     s->status, s->n_status, 0ull);
   HIPCHK(hipGetLastError());
-  s->lb_gen = 0;
   s->tab_gen = 0;
   s->cap = new_cap;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -283,27 +271,68 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
 }
 
 // ---- kernel launches, dispatched on W ------------------------------------------
+// Profiling phases (HIP events per launch).
+enum Phase { PH_COUNT = 0, PH_GATHER = 1, PH_PAIR = 2, PH_ECOUNT = 3, PH_EWRITE = 4, PH_MAP = 5,
+             PH_FINISH = 6, PH_OTHER = 7 };
+
+int chunks_for(u64 n) { return grid_for(n, 2048, kMaxParts); }
+
+int launch_finish(ppvm_sum* s, const FinishArgs& a) {
+  int rc;
+  if ((rc = prof_begin(s, PH_FINISH))) return rc;
+  hipLaunchKernelGGL(k_finish, dim3(1), dim3(kScanThreads), 0, s->stream, s->st, a);
+  return prof_end(s);
+}
+
 template <int W>
 int launch_rot(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
   int rc;
-  const int grid = grid_for(s->n_bound, kTile, kMaxGrid);
+  const u32 G = (u32)chunks_for(s->n_bound);
   WLRec<W>* wl = reinterpret_cast<WLRec<W>*>(s->wl);
-  u32 g1 = next_lb_gen(s, &rc);
-  if ((rc = prof_begin(s, 0))) return rc;
-  hipLaunchKernelGGL(k_rot_scan<W>, dim3(grid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
-                     s->coef[s->cur], s->cap, wl, s->st, in, out, it.rot, s->status, g1);
+  u64* planes = s->planes[s->cur];
+  double* coef = s->coef[s->cur];
+  if ((rc = prof_begin(s, PH_COUNT))) return rc;
+  hipLaunchKernelGGL(k_rot_count<W>, dim3(G), dim3(64), 0, s->stream, planes, coef, s->cap, s->st, in,
+                     it.rot, s->masks, s->part(0), G);
   if ((rc = prof_end(s))) return rc;
-  u32 tg = next_tab_gen(s);
-  if ((rc = prof_begin(s, 1))) return rc;
-  hipLaunchKernelGGL(k_rot_pair<W>, dim3(grid_for(s->n_bound, kBlock * 2, 4 * kMaxGrid)), dim3(kBlock), 0,
-                     s->stream, s->coef[s->cur], s->cap, wl, s->st, in, it.rot, s->table, s->tcap, tg,
-                     s->trunc_spec(), it.seq + 1);
+  FinishArgs fa{};
+  fa.mode = F_WORKLIST;
+  fa.in_slot = in;
+  fa.out_slot = out;
+  fa.G = G;
+  fa.cnt = s->part(0);
+  if ((rc = launch_finish(s, fa))) return rc;
+  if ((rc = prof_begin(s, PH_GATHER))) return rc;
+  hipLaunchKernelGGL(k_rot_gather<W>, dim3(G), dim3(64), 0, s->stream, planes, coef, s->cap, s->st, in,
+                     s->masks, s->part(0), wl, G);
   if ((rc = prof_end(s))) return rc;
-  u32 g2 = next_lb_gen(s, &rc);
-  if ((rc = prof_begin(s, 2))) return rc;
-  hipLaunchKernelGGL(k_rot_emit<W>, dim3(grid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
-                     s->coef[s->cur], s->cap, wl, s->st, in, out, it.rot, s->trunc_spec(), s->status, g2);
+  const u32 tg = next_tab_gen(s);
+  const u32 Gp = (u32)grid_for(s->n_bound, kBlock * 2, kMaxParts);
+  if ((rc = prof_begin(s, PH_PAIR))) return rc;
+  hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
+                     it.rot, s->table, s->tcap, tg, s->trunc_spec(), it.seq + 1, s->part(1), s->part(2));
+  if ((rc = prof_end(s))) return rc;
+  if ((rc = prof_begin(s, PH_ECOUNT))) return rc;
+  hipLaunchKernelGGL(k_emit_count<W>, dim3(G), dim3(64), 0, s->stream, coef, s->cap, wl, s->st, in, it.rot,
+                     s->trunc_spec(), s->masks, s->part(3), s->part(4), G);
+  if ((rc = prof_end(s))) return rc;
+  fa = FinishArgs{};
+  fa.mode = F_EMIT;
+  fa.in_slot = in;
+  fa.out_slot = out;
+  fa.G = G;
+  fa.cnt = s->part(3);
+  fa.Ga = G;
+  fa.aux0 = s->part(4);
+  fa.Gb = Gp;
+  fa.aux2 = s->part(1);
+  fa.aux3 = s->part(2);
+  fa.test_words = it.rot.test_words;
+  if ((rc = launch_finish(s, fa))) return rc;
+  if ((rc = prof_begin(s, PH_EWRITE))) return rc;
+  hipLaunchKernelGGL(k_emit_write<W>, dim3(G), dim3(64), 0, s->stream, planes, coef, s->cap, wl, s->st, in,
+                     it.rot, s->masks, s->part(3), G);
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
   return PPVM_OK;
@@ -313,11 +342,20 @@ template <int W>
 int launch_map(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
   int rc;
-  if ((rc = prof_begin(s, 3))) return rc;
-  hipLaunchKernelGGL(k_map<W>, dim3(grid_for(s->n_bound, kBlock * 4, 4 * kMaxGrid)), dim3(kBlock), 0,
-                     s->stream, s->planes[s->cur], s->coef[s->cur], s->cap, s->st, in, out, it.map,
-                     s->trunc_spec());
+  const u32 Gm = (u32)grid_for(s->n_bound, kBlock * 4, kMaxParts);
+  if ((rc = prof_begin(s, PH_MAP))) return rc;
+  hipLaunchKernelGGL(k_map<W>, dim3(Gm), dim3(kBlock), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
+                     s->cap, s->st, in, it.map, s->trunc_spec(), s->part(0), s->part(1));
   if ((rc = prof_end(s))) return rc;
+  FinishArgs fa{};
+  fa.mode = F_MAP;
+  fa.in_slot = in;
+  fa.out_slot = out;
+  fa.Ga = Gm;
+  fa.aux0 = s->part(0);
+  fa.aux1 = s->part(1);
+  fa.n_ops = it.map.n_ops;
+  if ((rc = launch_finish(s, fa))) return rc;
   HIPCHK(hipGetLastError());
   return PPVM_OK;
 }
@@ -325,10 +363,16 @@ int launch_map(ppvm_sum* s, const Issued& it) {
 template <int W>
 int launch_overlap(ppvm_sum* s) {
   int rc;
-  if ((rc = prof_begin(s, 4))) return rc;
+  const u32 in = (u32)(s->op_seq & 1);
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
   hipLaunchKernelGGL(k_overlap<W>, dim3(kOverlapGrid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
-                     s->coef[s->cur], s->cap, s->st, (u32)(s->op_seq & 1), s->partials);
+                     s->coef[s->cur], s->cap, s->st, in, s->partials);
   if ((rc = prof_end(s))) return rc;
+  FinishArgs fa{};
+  fa.mode = F_OVERLAP;
+  fa.Ga = kOverlapGrid;
+  fa.dpart = s->partials;
+  if ((rc = launch_finish(s, fa))) return rc;
   HIPCHK(hipGetLastError());
   return PPVM_OK;
 }
@@ -337,11 +381,21 @@ template <int W>
 int launch_compact(ppvm_sum* s) {
   int rc;
   const u32 in = (u32)(s->op_seq & 1), out = in ^ 1u;
-  u32 gen = next_lb_gen(s, &rc);
-  if ((rc = prof_begin(s, 5))) return rc;
-  hipLaunchKernelGGL(k_compact<W>, dim3(grid_for(s->n_bound, kTile, kMaxGrid)), dim3(kBlock), 0, s->stream,
-                     s->planes[s->cur], s->coef[s->cur], s->planes[s->cur ^ 1], s->coef[s->cur ^ 1], s->cap,
-                     s->st, in, out, s->status, gen);
+  const u32 G = (u32)chunks_for(s->n_bound);
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_compact_count, dim3(G), dim3(64), 0, s->stream, s->coef[s->cur], s->st, in, s->masks,
+                     s->part(0), G);
+  if ((rc = prof_end(s))) return rc;
+  FinishArgs fa{};
+  fa.mode = F_COMPACT;
+  fa.in_slot = in;
+  fa.out_slot = out;
+  fa.G = G;
+  fa.cnt = s->part(0);
+  if ((rc = launch_finish(s, fa))) return rc;
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_compact_write<W>, dim3(G), dim3(64), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
+                     s->planes[s->cur ^ 1], s->coef[s->cur ^ 1], s->cap, s->st, in, s->masks, s->part(0), G);
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
   s->op_seq += 1;
@@ -783,7 +837,8 @@ int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint,
   }
   if (hipMalloc(&s->st, sizeof(DevState)) != hipSuccess ||
       hipHostMalloc(&s->st_host, sizeof(DevState), hipHostMallocDefault) != hipSuccess ||
-      hipMalloc(&s->partials, kOverlapGrid * sizeof(double)) != hipSuccess) {
+      hipMalloc(&s->partials, kOverlapGrid * sizeof(double)) != hipSuccess ||
+      hipMalloc(&s->parts, 6 * (size_t)kMaxParts * sizeof(u64)) != hipSuccess) {
     (void)hipGetLastError();
     delete s;
     return fail(PPVM_ENOMEM, "device allocation of handle state failed");
@@ -835,6 +890,7 @@ int ppvm_destroy(ppvm_handle h) {
   if (h->st) (void)hipFree(h->st);
   if (h->st_host) (void)hipHostFree(h->st_host);
   if (h->partials) (void)hipFree(h->partials);
+  if (h->parts) (void)hipFree(h->parts);
   if (h->preserve) (void)hipFree(h->preserve);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);

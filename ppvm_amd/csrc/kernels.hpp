@@ -1,19 +1,23 @@
 // kernels.hpp — HIP kernels of the gfx950 Pauli-propagation engine.
 //
-// One rotation gate exp(-i theta/2 G) is three launches over the arena:
-//   k_rot_scan : stream the test word(s) of every entry; anticommuting live
-//                entries are compacted (ballot ranks + decoupled look-back, so
-//                the order is the arena order) into the worklist with their
-//                full key and coefficient.
-//   k_rot_pair : each worklist record inserts its pair key (key with the
-//                generator's pivot bit cleared) into a generation-tagged
-//                open-addressed table; the second member of a pair to arrive
-//                computes both merged coefficients
-//                   c_k' = c_k cos + eps_p c_p sin,  c_p' = c_p cos + eps_k c_k sin
-//                and writes them in place (with fused truncation).
-//   k_rot_emit : unpaired records scale their parent in place and append the
-//                child k ^ M (coefficient eps_k c_k sin) at the arena end, in
-//                worklist order, dropping children the truncation rejects.
+// One rotation gate exp(-i theta/2 G) runs as:
+//   k_rot_count  : stream the test word(s) of every entry; ballot masks of the
+//                  live anticommuting entries + one count per wave chunk.
+//   k_finish     : (one block) chunk counts -> offsets; worklist length.
+//   k_rot_gather : copy each anticommuting entry (full key, coefficient, arena
+//                  index) into the worklist, in arena order.
+//   k_rot_pair   : each worklist record inserts its pair key (key with the
+//                  generator's pivot bit cleared) into a generation-tagged
+//                  open-addressed table; the second member of a pair to arrive
+//                  computes both merged coefficients
+//                    c_k' = c_k cos + eps_p c_p sin,  c_p' = c_p cos + eps_k c_k sin
+//                  and writes them in place (with fused truncation).
+//   k_emit_count : unpaired records scale their parent in place and decide
+//                  whether their child k ^ M (coefficient eps_k c_k sin)
+//                  survives the truncation; masks + counts.
+//   k_finish     : offsets of the children; new counts and statistics.
+//   k_emit_write : append the surviving children at the arena end, in
+//                  worklist order.
 // Commuting entries are never touched: their only traffic is the test word.
 //
 // Floating point: products and sums use __dmul_rn / __dadd_rn (no contraction),
@@ -119,91 +123,127 @@ __device__ __forceinline__ void load_key(const u64* __restrict__ planes, u64 cap
   for (int p = 0; p < 2 * W; ++p) k[p] = planes[(u64)p * cap + i];
 }
 
-// ---- k_rot_scan --------------------------------------------------------------
+
+// ---- chunked passes ------------------------------------------------------------
+// Item range of wave chunk b when n items are split into G chunks whose sizes
+// are multiples of 64 (so every mask word belongs to one chunk).
+__device__ __forceinline__ void wave_chunk(u64 n, u32 G, u32 b, u64& lo, u64& hi) {
+  u64 per = (n + G - 1) / G;
+  per = (per + 63) & ~63ull;
+  lo = (u64)b * per;
+  if (lo > n) lo = n;
+  hi = lo + per;
+  if (hi > n) hi = n;
+}
+__device__ __forceinline__ u64 lanes_below(u32 lane) { return (1ull << lane) - 1ull; }
+
+// Anticommutation test of entry i from its test words only.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_scan(const u64* __restrict__ planes,
-                                                     const double* __restrict__ coef, u64 cap,
-                                                     WLRec<W>* __restrict__ wl, DevState* st,
-                                                     u32 in_slot, u32 out_slot, RotSpec g,
-                                                     u64* status, u32 gen) {
-  __shared__ u32 lds_cnt[kItems * kWaves];
-  __shared__ u32 lds_tile;
-  __shared__ u64 lds_pref;
-  __shared__ u32 lds_last;
-  if (ld_relaxed(&st->overflow)) return;
+__device__ __forceinline__ u32 rot_test(const u64* __restrict__ planes, u64 cap, u64 i,
+                                        const RotSpec& g, u32 wa, u32 wb, bool needx, bool needz) {
+  u32 r = 0;
+  if (needx) {
+    const u64 xa = planes[(u64)wa * cap + i];
+    r ^= bit_of(xa, g.a) & g.gaz;
+    if (g.two) {
+      const u64 xb = (wb == wa) ? xa : planes[(u64)wb * cap + i];
+      r ^= bit_of(xb, g.b) & g.gbz;
+    }
+  }
+  if (needz) {
+    const u64 za = planes[(u64)(W + wa) * cap + i];
+    r ^= bit_of(za, g.a) & g.gax;
+    if (g.two) {
+      const u64 zb = (wb == wa) ? za : planes[(u64)(W + wb) * cap + i];
+      r ^= bit_of(zb, g.b) & g.gbx;
+    }
+  }
+  return r;
+}
+
+constexpr int kUnroll = 4;  // mask words in flight per wave
+
+// ---- k_rot_count ----------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(64) void k_rot_count(const u64* __restrict__ planes,
+                                                  const double* __restrict__ coef, u64 cap,
+                                                  const DevState* __restrict__ st, u32 in_slot,
+                                                  RotSpec g, u64* __restrict__ masks,
+                                                  u64* __restrict__ cnt, u32 G) {
+  if (st->overflow) return;
+  const u32 lane = threadIdx.x;
   const u64 n = st->slot[in_slot].n_entries;
-  const u64 n_tiles = (n + kTile - 1) / kTile;
+  u64 lo, hi;
+  wave_chunk(n, G, blockIdx.x, lo, hi);
   const u32 wa = g.a >> 6, wb = g.b >> 6;
   const bool needx = (g.gaz | g.gbz) != 0u;
   const bool needz = (g.gax | g.gbx) != 0u;
-  while (true) {
-    long long tile = next_tile(&st->ticket[TK_SCAN], n_tiles, &lds_tile);
-    if (tile < 0) break;
-    bool anti[kItems];
-    double cv[kItems];
+  u64 count = 0;
+  for (u64 base = lo; base < hi; base += 64 * kUnroll) {
+    u32 r[kUnroll];
 #pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      u64 i = (u64)tile * kTile + (u64)it * kBlock + threadIdx.x;
-      anti[it] = false;
-      cv[it] = 0.0;
-      if (i < n) {
-        u32 r = 0;
-        if (needx) {
-          u64 xa = planes[(u64)wa * cap + i];
-          r ^= bit_of(xa, g.a) & g.gaz;
-          if (g.two) {
-            u64 xb = (wb == wa) ? xa : planes[(u64)wb * cap + i];
-            r ^= bit_of(xb, g.b) & g.gbz;
-          }
-        }
-        if (needz) {
-          u64 za = planes[(u64)(W + wa) * cap + i];
-          r ^= bit_of(za, g.a) & g.gax;
-          if (g.two) {
-            u64 zb = (wb == wa) ? za : planes[(u64)(W + wb) * cap + i];
-            r ^= bit_of(zb, g.b) & g.gbx;
-          }
-        }
-        if (r) {
-          cv[it] = coef[i];
-          anti[it] = !is_dead(cv[it]);
-        }
+    for (int u = 0; u < kUnroll; ++u) {
+      const u64 i = base + (u64)u * 64 + lane;
+      r[u] = (i < hi) ? rot_test<W>(planes, cap, i, g, wa, wb, needx, needz) : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const u64 i = base + (u64)u * 64 + lane;
+      bool anti = false;
+      if (r[u]) anti = !is_dead(coef[i]);
+      const u64 m = __ballot(anti);
+      const u64 wbase = base + (u64)u * 64;
+      if (wbase < hi) {
+        if (lane == 0) masks[wbase >> 6] = m;
+        count += (u64)__popcll(m);
       }
     }
-    u32 pos[kItems];
-    u32 total = tile_ranks(anti, pos, lds_cnt);
-    u64 prefix = lookback(status, gen, (u64)tile, total, &lds_pref);
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      if (!anti[it]) continue;
-      u64 i = (u64)tile * kTile + (u64)it * kBlock + threadIdx.x;
-      WLRec<W>& r = wl[prefix + pos[it]];
+  }
+  if (lane == 0) cnt[blockIdx.x] = count;
+}
+
+// ---- k_rot_gather ---------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(64) void k_rot_gather(const u64* __restrict__ planes,
+                                                   const double* __restrict__ coef, u64 cap,
+                                                   const DevState* __restrict__ st, u32 in_slot,
+                                                   const u64* __restrict__ masks,
+                                                   const u64* __restrict__ off,
+                                                   WLRec<W>* __restrict__ wl, u32 G) {
+  if (st->overflow) return;
+  const u32 lane = threadIdx.x;
+  const u64 n = st->slot[in_slot].n_entries;
+  u64 lo, hi;
+  wave_chunk(n, G, blockIdx.x, lo, hi);
+  u64 o = off[blockIdx.x];
+  for (u64 base = lo; base < hi; base += 64) {
+    const u64 m = masks[base >> 6];
+    if (m == 0) continue;
+    if ((m >> lane) & 1ull) {
+      const u64 i = base + lane;
+      WLRec<W>& r = wl[o + (u64)__popcll(m & lanes_below(lane))];
       u64 k[2 * W];
       load_key<W>(planes, cap, i, k);
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) r.k[p] = k[p];
-      r.c = cv[it];
+      r.c = coef[i];
       r.idx = (u32)i;
       r.flags = 0u;
     }
-  }
-  if (last_block(&st->done_count[DK_SCAN], &lds_last) && threadIdx.x == 0) {
-    u64 total = 0;
-    if (n_tiles) total = ld_acquire(&status[n_tiles - 1]) & ((1ull << 38) - 1);
-    st->n_wl = total;
-    st->op_kills = 0;
+    o += (u64)__popcll(m);
   }
 }
 
-// ---- k_rot_pair ---------------------------------------------------------------
+// ---- k_rot_pair -----------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, u64 cap,
                                                      WLRec<W>* __restrict__ wl, DevState* st,
                                                      u32 in_slot, RotSpec g, u64* table,
                                                      u64 tcap_max, u32 tgen, TruncSpec t,
-                                                     u64 op_tag) {
+                                                     u64 op_tag, u64* __restrict__ part_kills,
+                                                     u64* __restrict__ part_pairs) {
   __shared__ u64 lds_sum[kWaves];
-  if (ld_relaxed(&st->overflow)) return;
+  if (st->overflow) return;
   const u64 n_entries = st->slot[in_slot].n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) {
@@ -228,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
     u64 s = ld_relaxed(&table[pos]);
     while (true) {
       if ((u32)(s >> 40) != tgen) {
-        u64 old = cas_u64(&table[pos], s, mine);
+        const u64 old = cas_u64(&table[pos], s, mine);
         if (old == s) break;  // claimed: first member of its pair (so far)
         s = old;
         continue;
@@ -267,97 +307,212 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
       s = ld_relaxed(&table[pos]);
     }
   }
-  u64 bk = block_sum_u64(kills, lds_sum);
-  u64 bp = block_sum_u64(pairs, lds_sum);
+  const u64 bk = block_sum_u64(kills, lds_sum);
+  const u64 bp = block_sum_u64(pairs, lds_sum);
   if (threadIdx.x == 0) {
-    if (bk) add_u64(&st->op_kills, bk);
-    if (bp) add_u64(&st->pairs_total, bp);
+    part_kills[blockIdx.x] = bk;
+    part_pairs[blockIdx.x] = bp;
   }
 }
 
-// ---- k_rot_emit ---------------------------------------------------------------
+// ---- k_emit_count -----------------------------------------------------------------
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_emit(u64* __restrict__ planes,
-                                                     double* __restrict__ coef, u64 cap,
-                                                     const WLRec<W>* __restrict__ wl,
-                                                     DevState* st, u32 in_slot, u32 out_slot,
-                                                     RotSpec g, TruncSpec t, u64* status,
-                                                     u32 gen) {
-  __shared__ u32 lds_cnt[kItems * kWaves];
-  __shared__ u32 lds_tile;
-  __shared__ u64 lds_pref;
-  __shared__ u64 lds_sum[kWaves];
-  __shared__ u32 lds_last;
-  if (ld_relaxed(&st->overflow)) return;
+__global__ __launch_bounds__(64) void k_emit_count(double* __restrict__ coef, u64 cap,
+                                                   const WLRec<W>* __restrict__ wl,
+                                                   const DevState* __restrict__ st, u32 in_slot,
+                                                   RotSpec g, TruncSpec t, u64* __restrict__ masks,
+                                                   u64* __restrict__ cnt, u64* __restrict__ kills_out,
+                                                   u32 G) {
+  if (st->overflow) return;
   const u64 n_entries = st->slot[in_slot].n_entries;
   const u64 n_wl = st->n_wl;
-  if (n_entries + n_wl > cap) return;  // k_rot_pair flagged the overflow
-  const u64 n_tiles = (n_wl + kTile - 1) / kTile;
-  while (true) {
-    long long tile = next_tile(&st->ticket[TK_EMIT], n_tiles, &lds_tile);
-    if (tile < 0) break;
-    bool child[kItems];
-    u64 kills = 0;
+  if (n_entries + n_wl > cap) return;
+  const u32 lane = threadIdx.x;
+  u64 lo, hi;
+  wave_chunk(n_wl, G, blockIdx.x, lo, hi);
+  u64 count = 0, kills = 0;
+  for (u64 base = lo; base < hi; base += 64) {
+    const u64 w = base + lane;
+    bool keep = false;
+    if (w < hi && wl[w].flags == 0u) {
+      u64 k[2 * W];
 #pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      u64 w = (u64)tile * kTile + (u64)it * kBlock + threadIdx.x;
-      child[it] = false;
-      if (w < n_wl && wl[w].flags == 0u) {
-        u64 k[2 * W];
-#pragma unroll
-        for (int p = 0; p < 2 * W; ++p) k[p] = wl[w].k[p];
-        const double c = wl[w].c;
-        const double eps = rot_eps<W>(k, g);
-        const double cp = __dmul_rn(c, g.cs);
-        const double cc = __dmul_rn(c, eps * g.sn);
-        bool dp = false, keep = true;
-        if (g.trunc) {
-          dp = drop_term<W>(k, cp, t);
-          rot_flip<W>(k, g);
-          keep = !drop_term<W>(k, cc, t);
-        }
-        coef[wl[w].idx] = dp ? dead_value() : cp;
-        kills += (u64)dp;
-        child[it] = keep;
+      for (int p = 0; p < 2 * W; ++p) k[p] = wl[w].k[p];
+      const double c = wl[w].c;
+      const double eps = rot_eps<W>(k, g);
+      const double cp = __dmul_rn(c, g.cs);
+      const double cc = __dmul_rn(c, eps * g.sn);
+      bool dp = false;
+      keep = true;
+      if (g.trunc) {
+        dp = drop_term<W>(k, cp, t);
+        rot_flip<W>(k, g);
+        keep = !drop_term<W>(k, cc, t);
       }
+      coef[wl[w].idx] = dp ? dead_value() : cp;
+      kills += (u64)dp;
     }
-    u64 bk = block_sum_u64(kills, lds_sum);
-    if (threadIdx.x == 0 && bk) add_u64(&st->op_kills, bk);
-    u32 pos[kItems];
-    u32 total = tile_ranks(child, pos, lds_cnt);
-    u64 prefix = lookback(status, gen, (u64)tile, total, &lds_pref);
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      if (!child[it]) continue;
-      u64 w = (u64)tile * kTile + (u64)it * kBlock + threadIdx.x;
+    const u64 m = __ballot(keep);
+    if (lane == 0) masks[base >> 6] = m;
+    count += (u64)__popcll(m);
+  }
+  for (int off = 32; off > 0; off >>= 1) kills += __shfl_down(kills, off, 64);
+  if (lane == 0) {
+    cnt[blockIdx.x] = count;
+    kills_out[blockIdx.x] = kills;
+  }
+}
+
+// ---- k_emit_write -----------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(64) void k_emit_write(u64* __restrict__ planes, double* __restrict__ coef,
+                                                   u64 cap, const WLRec<W>* __restrict__ wl,
+                                                   const DevState* __restrict__ st, u32 in_slot,
+                                                   RotSpec g, const u64* __restrict__ masks,
+                                                   const u64* __restrict__ off, u32 G) {
+  if (st->overflow) return;
+  const u64 n_entries = st->slot[in_slot].n_entries;
+  const u64 n_wl = st->n_wl;
+  if (n_entries + n_wl > cap) return;
+  const u32 lane = threadIdx.x;
+  u64 lo, hi;
+  wave_chunk(n_wl, G, blockIdx.x, lo, hi);
+  u64 o = n_entries + off[blockIdx.x];
+  for (u64 base = lo; base < hi; base += 64) {
+    const u64 m = masks[base >> 6];
+    if (m == 0) continue;
+    if ((m >> lane) & 1ull) {
+      const u64 w = base + lane;
       u64 k[2 * W];
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) k[p] = wl[w].k[p];
       const double eps = rot_eps<W>(k, g);
       const double cc = __dmul_rn(wl[w].c, eps * g.sn);
       rot_flip<W>(k, g);
-      const u64 dst = n_entries + prefix + pos[it];
+      const u64 dst = o + (u64)__popcll(m & lanes_below(lane));
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) planes[(u64)p * cap + dst] = k[p];
       coef[dst] = cc;
     }
+    o += (u64)__popcll(m);
   }
-  if (last_block(&st->done_count[DK_EMIT], &lds_last) && threadIdx.x == 0) {
-    u64 appended = 0;
-    if (n_tiles) appended = ld_acquire(&status[n_tiles - 1]) & ((1ull << 38) - 1);
-    const u64 kills = ld_relaxed(&st->op_kills);
-    const u64 dead_in = st->slot[in_slot].n_dead;
-    st->slot[out_slot].n_entries = n_entries + appended;
-    st->slot[out_slot].n_dead = dead_in + kills;
-    st->op_kills = 0;
-    st->term_updates += n_entries - dead_in;
+}
+
+// ---- k_finish: single-block scan / reduction / bookkeeping ------------------------
+enum FinishMode : u32 { F_WORKLIST = 0, F_EMIT = 1, F_MAP = 2, F_COMPACT = 3, F_OVERLAP = 4 };
+struct FinishArgs {
+  u32 mode;
+  u32 in_slot, out_slot;
+  u32 G;               // entries of `cnt` to scan (0: none)
+  u32 Ga;              // entries of aux0 / aux1 to sum
+  u32 Gb;              // entries of aux2 / aux3 to sum
+  u32 n_ops;           // map program length (F_MAP)
+  u32 test_words;      // rotation test words (F_EMIT)
+  u64* cnt;            // in: counts, out: exclusive offsets
+  const u64* aux0;     // kills (emit count / map)
+  const u64* aux1;     // updates (map)
+  const u64* aux2;     // kills (pair)
+  const u64* aux3;     // pairs (pair)
+  const double* dpart; // overlap partials (F_OVERLAP)
+};
+
+__device__ __forceinline__ u64 block_reduce_1024(u64 v, u64* lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  u64 t = 0;
+  for (u32 w = 0; w < blockDim.x / 64; ++w) t += lds[w];
+  __syncthreads();
+  return t;  // valid in every thread
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_finish(DevState* st, FinishArgs a) {
+  __shared__ u64 lds[kScanThreads / 64];
+  __shared__ u64 lds_scan[kScanThreads];
+  __shared__ double lds_d[kScanThreads];
+  if (a.mode != F_OVERLAP && a.mode != F_COMPACT && st->overflow) return;
+  const u32 tid = threadIdx.x;
+  if (a.mode == F_OVERLAP) {
+    // ordered sum of per-block partials (deterministic)
+    double v = 0.0;
+    const u32 per = (a.Ga + kScanThreads - 1) / kScanThreads;
+    for (u32 j = tid * per; j < (tid + 1) * per && j < a.Ga; ++j) v = __dadd_rn(v, a.dpart[j]);
+    lds_d[tid] = v;
+    __syncthreads();
+    if (tid == 0) {
+      double tot = 0.0;
+      for (u32 j = 0; j < kScanThreads; ++j) tot = __dadd_rn(tot, lds_d[j]);
+      st->overlap = tot;
+    }
+    return;
+  }
+  // exclusive scan of cnt[0..G)
+  u64 total = 0;
+  if (a.G) {
+    const u32 per = (a.G + kScanThreads - 1) / kScanThreads;
+    const u32 j0 = tid * per;
+    u64 local = 0;
+    for (u32 j = j0; j < j0 + per && j < a.G; ++j) local += a.cnt[j];
+    lds_scan[tid] = local;
+    __syncthreads();
+    // Hillis-Steele inclusive scan over 1024 partial sums
+    for (u32 d = 1; d < kScanThreads; d <<= 1) {
+      const u64 add = tid >= d ? lds_scan[tid - d] : 0ull;
+      __syncthreads();
+      lds_scan[tid] += add;
+      __syncthreads();
+    }
+    total = lds_scan[kScanThreads - 1];
+    u64 run = lds_scan[tid] - local;
+    for (u32 j = j0; j < j0 + per && j < a.G; ++j) {
+      const u64 c = a.cnt[j];
+      a.cnt[j] = run;
+      run += c;
+    }
+  }
+  u64 s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  for (u32 j = tid; j < a.Ga; j += kScanThreads) {
+    if (a.aux0) s0 += a.aux0[j];
+    if (a.aux1) s1 += a.aux1[j];
+  }
+  for (u32 j = tid; j < a.Gb; j += kScanThreads) {
+    if (a.aux2) s2 += a.aux2[j];
+    if (a.aux3) s3 += a.aux3[j];
+  }
+  s0 = block_reduce_1024(s0, lds);
+  s1 = block_reduce_1024(s1, lds);
+  s2 = block_reduce_1024(s2, lds);
+  s3 = block_reduce_1024(s3, lds);
+  if (tid != 0) return;
+  const CountSlot in = st->slot[a.in_slot];
+  if (a.mode == F_WORKLIST) {
+    st->n_wl = total;
+  } else if (a.mode == F_EMIT) {
+    const u64 kills = s0 + s2;
+    st->slot[a.out_slot].n_entries = in.n_entries + total;
+    st->slot[a.out_slot].n_dead = in.n_dead + kills;
+    st->term_updates += in.n_entries - in.n_dead;
     st->gates += 1;
     st->rotations += 1;
-    st->scanned += n_entries;
-    st->scan_words += n_entries * g.test_words;
-    st->wl_total += n_wl;
-    st->children_total += appended;
+    st->scanned += in.n_entries;
+    st->scan_words += in.n_entries * a.test_words;
+    st->wl_total += st->n_wl;
+    st->pairs_total += s3;
+    st->children_total += total;
     st->kills_total += kills;
+  } else if (a.mode == F_MAP) {
+    st->slot[a.out_slot].n_entries = in.n_entries;
+    st->slot[a.out_slot].n_dead = in.n_dead + s0;
+    st->term_updates += s1;
+    st->gates += a.n_ops;
+    st->kills_total += s0;
+    st->map_entries += in.n_entries;
+    st->map_launches += 1;
+  } else if (a.mode == F_COMPACT) {
+    st->slot[a.out_slot].n_entries = total;
+    st->slot[a.out_slot].n_dead = 0;
   }
 }
 
@@ -417,11 +572,12 @@ __device__ __forceinline__ void map_apply(u64 (&k)[2 * W], double& c, const MapO
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ planes,
                                                 double* __restrict__ coef, u64 cap,
-                                                DevState* st, u32 in_slot, u32 out_slot,
-                                                MapProgram prog, TruncSpec t) {
+                                                const DevState* __restrict__ st, u32 in_slot,
+                                                MapProgram prog, TruncSpec t,
+                                                u64* __restrict__ part_kills,
+                                                u64* __restrict__ part_updates) {
   __shared__ u64 lds_sum[kWaves];
-  __shared__ u32 lds_last;
-  if (ld_relaxed(&st->overflow)) return;
+  if (st->overflow) return;
   const u64 n = st->slot[in_slot].n_entries;
   u64 kills = 0, updates = 0;
   const u64 stride = (u64)gridDim.x * kBlock;
@@ -450,24 +606,11 @@ __global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ planes,
       if (k[p] != k0[p]) planes[(u64)p * cap + i] = k[p];
     if (__double_as_longlong(c) != __double_as_longlong(c0)) coef[i] = c;
   }
-  u64 bk = block_sum_u64(kills, lds_sum);
-  u64 bu = block_sum_u64(updates, lds_sum);
+  const u64 bk = block_sum_u64(kills, lds_sum);
+  const u64 bu = block_sum_u64(updates, lds_sum);
   if (threadIdx.x == 0) {
-    if (bk) add_u64(&st->op_kills, bk);
-    if (bu) add_u64(&st->op_updates, bu);
-  }
-  if (last_block(&st->done_count[DK_MAP], &lds_last) && threadIdx.x == 0) {
-    const u64 kills_all = ld_relaxed(&st->op_kills);
-    const u64 upd = ld_relaxed(&st->op_updates);
-    st->slot[out_slot].n_entries = n;
-    st->slot[out_slot].n_dead = st->slot[in_slot].n_dead + kills_all;
-    st->op_kills = 0;
-    st->op_updates = 0;
-    st->term_updates += upd;
-    st->gates += prog.n_ops;
-    st->kills_total += kills_all;
-    st->map_entries += n;
-    st->map_launches += 1;
+    part_kills[blockIdx.x] = bk;
+    part_updates[blockIdx.x] = bu;
   }
 }
 
@@ -475,10 +618,9 @@ __global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ planes,
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ planes,
                                                     const double* __restrict__ coef, u64 cap,
-                                                    DevState* st, u32 in_slot,
-                                                    double* partials) {
+                                                    const DevState* __restrict__ st, u32 in_slot,
+                                                    double* __restrict__ partials) {
   __shared__ double lds_d[kBlock];
-  __shared__ u32 lds_last;
   const u64 n = st->slot[in_slot].n_entries;
   double acc = 0.0;
   const u64 stride = (u64)gridDim.x * kBlock;
@@ -498,60 +640,50 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ plan
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[blockIdx.x] = lds_d[0];
-  if (last_block(&st->done_count[DK_OVERLAP], &lds_last) && threadIdx.x == 0) {
-    double tot = 0.0;
-    for (u32 b = 0; b < gridDim.x; ++b) tot = __dadd_rn(tot, partials[b]);
-    st->overlap = tot;
-  }
 }
 
-// ---- k_compact: stable removal of dead entries into the other arena ------------
-template <int W>
-__global__ __launch_bounds__(kBlock) void k_compact(const u64* __restrict__ src_planes,
-                                                    const double* __restrict__ src_coef,
-                                                    u64* __restrict__ dst_planes,
-                                                    double* __restrict__ dst_coef, u64 cap,
-                                                    DevState* st, u32 in_slot, u32 out_slot,
-                                                    u64* status, u32 gen) {
-  __shared__ u32 lds_cnt[kItems * kWaves];
-  __shared__ u32 lds_tile;
-  __shared__ u64 lds_pref;
-  __shared__ u32 lds_last;
+// ---- dead-entry compaction into the other arena --------------------------------
+__global__ __launch_bounds__(64) void k_compact_count(const double* __restrict__ coef,
+                                                      const DevState* __restrict__ st, u32 in_slot,
+                                                      u64* __restrict__ masks, u64* __restrict__ cnt,
+                                                      u32 G) {
+  const u32 lane = threadIdx.x;
   const u64 n = st->slot[in_slot].n_entries;
-  const u64 n_tiles = (n + kTile - 1) / kTile;
-  while (true) {
-    long long tile = next_tile(&st->ticket[TK_COMPACT], n_tiles, &lds_tile);
-    if (tile < 0) break;
-    bool live[kItems];
-    double cv[kItems];
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      u64 i = (u64)tile * kTile + (u64)it * kBlock + threadIdx.x;
-      live[it] = false;
-      cv[it] = 0.0;
-      if (i < n) {
-        cv[it] = src_coef[i];
-        live[it] = !is_dead(cv[it]);
-      }
-    }
-    u32 pos[kItems];
-    u32 total = tile_ranks(live, pos, lds_cnt);
-    u64 prefix = lookback(status, gen, (u64)tile, total, &lds_pref);
-#pragma unroll
-    for (int it = 0; it < kItems; ++it) {
-      if (!live[it]) continue;
-      u64 i = (u64)tile * kTile + (u64)it * kBlock + threadIdx.x;
-      u64 dst = prefix + pos[it];
-#pragma unroll
-      for (int p = 0; p < 2 * W; ++p) dst_planes[(u64)p * cap + dst] = src_planes[(u64)p * cap + i];
-      dst_coef[dst] = cv[it];
-    }
+  u64 lo, hi;
+  wave_chunk(n, G, blockIdx.x, lo, hi);
+  u64 count = 0;
+  for (u64 base = lo; base < hi; base += 64) {
+    const u64 i = base + lane;
+    const bool live = i < hi && !is_dead(coef[i]);
+    const u64 m = __ballot(live);
+    if (lane == 0) masks[base >> 6] = m;
+    count += (u64)__popcll(m);
   }
-  if (last_block(&st->done_count[DK_COMPACT], &lds_last) && threadIdx.x == 0) {
-    u64 total = 0;
-    if (n_tiles) total = ld_acquire(&status[n_tiles - 1]) & ((1ull << 38) - 1);
-    st->slot[out_slot].n_entries = total;
-    st->slot[out_slot].n_dead = 0;
+  if (lane == 0) cnt[blockIdx.x] = count;
+}
+
+template <int W>
+__global__ __launch_bounds__(64) void k_compact_write(const u64* __restrict__ sp,
+                                                      const double* __restrict__ sc,
+                                                      u64* __restrict__ dp, double* __restrict__ dc,
+                                                      u64 cap, const DevState* __restrict__ st,
+                                                      u32 in_slot, const u64* __restrict__ masks,
+                                                      const u64* __restrict__ off, u32 G) {
+  const u32 lane = threadIdx.x;
+  const u64 n = st->slot[in_slot].n_entries;
+  u64 lo, hi;
+  wave_chunk(n, G, blockIdx.x, lo, hi);
+  u64 o = off[blockIdx.x];
+  for (u64 base = lo; base < hi; base += 64) {
+    const u64 m = masks[base >> 6];
+    if ((m >> lane) & 1ull) {
+      const u64 i = base + lane;
+      const u64 dst = o + (u64)__popcll(m & lanes_below(lane));
+#pragma unroll
+      for (int p = 0; p < 2 * W; ++p) dp[(u64)p * cap + dst] = sp[(u64)p * cap + i];
+      dc[dst] = sc[i];
+    }
+    o += (u64)__popcll(m);
   }
 }
 
