@@ -54,8 +54,7 @@ def phase_bytes(cnt: dict, W: int) -> dict:
     term, R = 16W + 16 B per worklist record, N = scanned entries, N_a =
     anticommuting terms, N_p = merged pairs, N_c = appended children):
 
-    k_rot_count  : 8 B per test word of every entry + 8 B coefficient of each
-                   anticommuting candidate + N/8 B of ballot masks.
+    k_rot_count  : 8 B per test word of every entry + N/8 B of ballot masks.
     k_rot_gather : N/8 B masks + N_a (S + R): key and coefficient read, record written.
     k_rot_pair   : N_a R records read + 2 N_p 8 B coefficients written.
     k_emit_count : N_a R records read + (N_a - 2 N_p) 8 B coefficients written + N_a/8 masks.
@@ -68,7 +67,7 @@ def phase_bytes(cnt: dict, W: int) -> dict:
     npair = cnt["pairs_total"]
     nch = cnt["children_total"]
     return {
-        "k_rot_count": 8 * cnt["scan_words"] + 8 * na + N / 8,
+        "k_rot_count": 8 * cnt["scan_words"] + N / 8,
         "k_rot_gather": N / 8 + na * (S + R),
         "k_rot_pair": na * R + 2 * npair * 8,
         "k_emit_count": na * R + (na - 2 * npair) * 8 + na / 8,

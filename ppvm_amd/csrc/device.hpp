@@ -9,14 +9,14 @@
 //                           record = full key + coefficient + arena index).
 //   table  : u64[tcap]      open-addressed pair table (generation-tagged slots).
 //   masks  : u64[cap/64]    per-pass ballot bitmasks (bit j of word w = item 64w+j)
-//   parts  : u64[6][8192]   per-chunk counts / offsets and per-block partial sums.
+//   parts  : u64[6][32768]  per-chunk counts and per-block partial sums.
 //
 // Stream compaction (worklist, children, dead-entry removal) is done without
 // atomics or inter-block waiting: a pass over n items is split into G
-// contiguous wave-sized chunks; a count kernel writes each chunk's ballot
-// masks and count, a single-block kernel turns counts into offsets, and a
-// write kernel re-reads only the masks to place every selected item.  The
-// output order is the input order, so results are deterministic.
+// contiguous chunks; a count kernel writes each chunk's ballot masks and
+// count, and a write kernel re-reads only the masks, takes its chunk offset
+// from the counts of the earlier chunks and ranks items inside the chunk with
+// an LDS scan.  The output order is the input order (deterministic).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,7 +29,7 @@ namespace ppvm {
 
 constexpr int kBlock = 256;        // threads per block (4 wave64) for grid-stride kernels
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxParts = 8192;    // max wave-chunks / blocks whose partials are reduced
+constexpr int kMaxParts = 32768;   // max chunks / blocks whose partials are reduced
 constexpr int kScanThreads = 1024; // single-block scan / reduction kernels
 
 // Dead-entry marker: a quiet NaN with a fixed payload (sign ignored).

@@ -275,7 +275,15 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
 enum Phase { PH_COUNT = 0, PH_GATHER = 1, PH_PAIR = 2, PH_ECOUNT = 3, PH_EWRITE = 4, PH_MAP = 5,
              PH_FINISH = 6, PH_OTHER = 7 };
 
-int chunks_for(u64 n) { return grid_for(n, 2048, kMaxParts); }
+// Block chunks for a pass over at most n_bound items in an arena of capacity
+// cap: at most kChunkMaxWords mask words per chunk, ~32K items per chunk.
+u32 chunks_for(const ppvm_sum* s) {
+  const u64 per_max = (u64)kChunkMaxWords * 64;
+  u64 g = std::max<u64>((s->cap + per_max - 1) / per_max, (s->n_bound + 32767) / 32768);
+  if (g < 1) g = 1;
+  if (g > (u64)kMaxParts) g = kMaxParts;
+  return (u32)g;
+}
 
 int launch_finish(ppvm_sum* s, const FinishArgs& a) {
   int rc;
@@ -288,23 +296,16 @@ template <int W>
 int launch_rot(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
   int rc;
-  const u32 G = (u32)chunks_for(s->n_bound);
+  const u32 G = chunks_for(s);
   WLRec<W>* wl = reinterpret_cast<WLRec<W>*>(s->wl);
   u64* planes = s->planes[s->cur];
   double* coef = s->coef[s->cur];
   if ((rc = prof_begin(s, PH_COUNT))) return rc;
-  hipLaunchKernelGGL(k_rot_count<W>, dim3(G), dim3(64), 0, s->stream, planes, coef, s->cap, s->st, in,
+  hipLaunchKernelGGL(k_rot_count<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, s->cap, s->st, in,
                      it.rot, s->masks, s->part(0), G);
   if ((rc = prof_end(s))) return rc;
-  FinishArgs fa{};
-  fa.mode = F_WORKLIST;
-  fa.in_slot = in;
-  fa.out_slot = out;
-  fa.G = G;
-  fa.cnt = s->part(0);
-  if ((rc = launch_finish(s, fa))) return rc;
   if ((rc = prof_begin(s, PH_GATHER))) return rc;
-  hipLaunchKernelGGL(k_rot_gather<W>, dim3(G), dim3(64), 0, s->stream, planes, coef, s->cap, s->st, in,
+  hipLaunchKernelGGL(k_rot_gather<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, coef, s->cap, s->st, in,
                      s->masks, s->part(0), wl, G);
   if ((rc = prof_end(s))) return rc;
   const u32 tg = next_tab_gen(s);
@@ -314,25 +315,12 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
                      it.rot, s->table, s->tcap, tg, s->trunc_spec(), it.seq + 1, s->part(1), s->part(2));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_ECOUNT))) return rc;
-  hipLaunchKernelGGL(k_emit_count<W>, dim3(G), dim3(64), 0, s->stream, coef, s->cap, wl, s->st, in, it.rot,
-                     s->trunc_spec(), s->masks, s->part(3), s->part(4), G);
+  hipLaunchKernelGGL(k_emit_count<W>, dim3(G), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
+                     it.rot, s->trunc_spec(), s->masks, s->part(3), s->part(4), G);
   if ((rc = prof_end(s))) return rc;
-  fa = FinishArgs{};
-  fa.mode = F_EMIT;
-  fa.in_slot = in;
-  fa.out_slot = out;
-  fa.G = G;
-  fa.cnt = s->part(3);
-  fa.Ga = G;
-  fa.aux0 = s->part(4);
-  fa.Gb = Gp;
-  fa.aux2 = s->part(1);
-  fa.aux3 = s->part(2);
-  fa.test_words = it.rot.test_words;
-  if ((rc = launch_finish(s, fa))) return rc;
   if ((rc = prof_begin(s, PH_EWRITE))) return rc;
-  hipLaunchKernelGGL(k_emit_write<W>, dim3(G), dim3(64), 0, s->stream, planes, coef, s->cap, wl, s->st, in,
-                     it.rot, s->masks, s->part(3), G);
+  hipLaunchKernelGGL(k_emit_write<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, coef, s->cap, wl, s->st,
+                     in, out, it.rot, s->masks, s->part(3), s->part(4), s->part(1), s->part(2), G, Gp);
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
   return PPVM_OK;
@@ -381,21 +369,15 @@ template <int W>
 int launch_compact(ppvm_sum* s) {
   int rc;
   const u32 in = (u32)(s->op_seq & 1), out = in ^ 1u;
-  const u32 G = (u32)chunks_for(s->n_bound);
+  const u32 G = chunks_for(s);
   if ((rc = prof_begin(s, PH_OTHER))) return rc;
-  hipLaunchKernelGGL(k_compact_count, dim3(G), dim3(64), 0, s->stream, s->coef[s->cur], s->st, in, s->masks,
-                     s->part(0), G);
+  hipLaunchKernelGGL(k_compact_count, dim3(G), dim3(kBlock), 0, s->stream, s->coef[s->cur], s->st, in,
+                     s->masks, s->part(0), G);
   if ((rc = prof_end(s))) return rc;
-  FinishArgs fa{};
-  fa.mode = F_COMPACT;
-  fa.in_slot = in;
-  fa.out_slot = out;
-  fa.G = G;
-  fa.cnt = s->part(0);
-  if ((rc = launch_finish(s, fa))) return rc;
   if ((rc = prof_begin(s, PH_OTHER))) return rc;
-  hipLaunchKernelGGL(k_compact_write<W>, dim3(G), dim3(64), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
-                     s->planes[s->cur ^ 1], s->coef[s->cur ^ 1], s->cap, s->st, in, s->masks, s->part(0), G);
+  hipLaunchKernelGGL(k_compact_write<W>, dim3(G), dim3(kBlock), 0, s->stream, s->planes[s->cur],
+                     s->coef[s->cur], s->planes[s->cur ^ 1], s->coef[s->cur ^ 1], s->cap, s->st, in, out,
+                     s->masks, s->part(0), G);
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
   s->op_seq += 1;

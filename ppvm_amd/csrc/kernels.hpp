@@ -2,10 +2,10 @@
 //
 // One rotation gate exp(-i theta/2 G) runs as:
 //   k_rot_count  : stream the test word(s) of every entry; ballot masks of the
-//                  live anticommuting entries + one count per wave chunk.
-//   k_finish     : (one block) chunk counts -> offsets; worklist length.
+//                  anticommuting entries + one count per block chunk.
 //   k_rot_gather : copy each anticommuting entry (full key, coefficient, arena
-//                  index) into the worklist, in arena order.
+//                  index) into the worklist, in arena order (dead entries are
+//                  flagged, not copied out).
 //   k_rot_pair   : each worklist record inserts its pair key (key with the
 //                  generator's pivot bit cleared) into a generation-tagged
 //                  open-addressed table; the second member of a pair to arrive
@@ -15,9 +15,8 @@
 //   k_emit_count : unpaired records scale their parent in place and decide
 //                  whether their child k ^ M (coefficient eps_k c_k sin)
 //                  survives the truncation; masks + counts.
-//   k_finish     : offsets of the children; new counts and statistics.
 //   k_emit_write : append the surviving children at the arena end, in
-//                  worklist order.
+//                  worklist order; publish the new counts and statistics.
 // Commuting entries are never touched: their only traffic is the test word.
 //
 // Floating point: products and sums use __dmul_rn / __dadd_rn (no contraction),
@@ -125,17 +124,72 @@ __device__ __forceinline__ void load_key(const u64* __restrict__ planes, u64 cap
 
 
 // ---- chunked passes ------------------------------------------------------------
-// Item range of wave chunk b when n items are split into G chunks whose sizes
-// are multiples of 64 (so every mask word belongs to one chunk).
-__device__ __forceinline__ void wave_chunk(u64 n, u32 G, u32 b, u64& lo, u64& hi) {
+// A pass over n items is split into G contiguous block chunks whose sizes are
+// multiples of 256 (4 mask words) and at most kChunkMaxWords mask words; the
+// host picks G >= capacity / (64 kChunkMaxWords).  A count kernel writes each
+// chunk's ballot masks and count; the matching write kernel derives its
+// chunk's offset from the counts of the chunks before it and ranks the items
+// inside the chunk with a scan of the mask popcounts in LDS.
+constexpr int kChunkMaxWords = 1024;
+
+__device__ __forceinline__ void block_chunk(u64 n, u32 G, u32 b, u64& lo, u64& hi) {
   u64 per = (n + G - 1) / G;
-  per = (per + 63) & ~63ull;
+  per = (per + 255) & ~255ull;
+  if (per > (u64)kChunkMaxWords * 64) per = (u64)kChunkMaxWords * 64;
   lo = (u64)b * per;
   if (lo > n) lo = n;
   hi = lo + per;
   if (hi > n) hi = n;
 }
 __device__ __forceinline__ u64 lanes_below(u32 lane) { return (1ull << lane) - 1ull; }
+
+// Sum of v over the block, returned to every thread.
+__device__ __forceinline__ u64 block_sum_all(u64 v, u64* lds) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  u64 t = 0;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) t += lds[w];
+  __syncthreads();
+  return t;
+}
+
+// Sum of cnt[0..b) (the chunk's offset), returned to every thread.
+__device__ __forceinline__ u64 chunk_offset(const u64* __restrict__ cnt, u32 b, u64* lds) {
+  u64 s = 0;
+  for (u32 j = threadIdx.x; j < b; j += kBlock) s += cnt[j];
+  return block_sum_all(s, lds);
+}
+
+// Loads the chunk's nw mask words into lm and writes their exclusive popcount
+// prefix into loff.
+__device__ __forceinline__ void chunk_mask_scan(const u64* __restrict__ masks, u64 w0, u32 nw,
+                                                u64* lm, u32* loff, u32* lt) {
+  const u32 tid = threadIdx.x;
+  const u32 q = (nw + kBlock - 1) / kBlock;
+  const u32 j0 = tid * q;
+  u32 local = 0;
+  for (u32 j = j0; j < j0 + q && j < nw; ++j) {
+    const u64 m = masks[w0 + j];
+    lm[j] = m;
+    loff[j] = local;
+    local += (u32)__popcll(m);
+  }
+  lt[tid] = local;
+  __syncthreads();
+  for (u32 d = 1; d < (u32)kBlock; d <<= 1) {
+    const u32 add = tid >= d ? lt[tid - d] : 0u;
+    __syncthreads();
+    lt[tid] += add;
+    __syncthreads();
+  }
+  const u32 excl = lt[tid] - local;
+  for (u32 j = j0; j < j0 + q && j < nw; ++j) loff[j] += excl;
+  __syncthreads();
+}
 
 // Anticommutation test of entry i from its test words only.
 template <int W>
@@ -164,73 +218,86 @@ __device__ __forceinline__ u32 rot_test(const u64* __restrict__ planes, u64 cap,
 constexpr int kUnroll = 4;  // mask words in flight per wave
 
 // ---- k_rot_count ----------------------------------------------------------------
+// Pure test-word stream: dead entries are filtered later (their records are
+// flagged by k_rot_gather, which reads every candidate's coefficient anyway).
 template <int W>
-__global__ __launch_bounds__(64) void k_rot_count(const u64* __restrict__ planes,
-                                                  const double* __restrict__ coef, u64 cap,
-                                                  const DevState* __restrict__ st, u32 in_slot,
-                                                  RotSpec g, u64* __restrict__ masks,
-                                                  u64* __restrict__ cnt, u32 G) {
+__global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ planes, u64 cap,
+                                                      const DevState* __restrict__ st, u32 in_slot,
+                                                      RotSpec g, u64* __restrict__ masks,
+                                                      u64* __restrict__ cnt, u32 G) {
+  __shared__ u64 lds[kWaves];
   if (st->overflow) return;
-  const u32 lane = threadIdx.x;
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
   u64 lo, hi;
-  wave_chunk(n, G, blockIdx.x, lo, hi);
+  block_chunk(n, G, blockIdx.x, lo, hi);
+  const u32 nw = (u32)((hi - lo + 63) / 64);
   const u32 wa = g.a >> 6, wb = g.b >> 6;
   const bool needx = (g.gaz | g.gbz) != 0u;
   const bool needz = (g.gax | g.gbx) != 0u;
   u64 count = 0;
-  for (u64 base = lo; base < hi; base += 64 * kUnroll) {
+  for (u32 j = wave; j < nw; j += kWaves * kUnroll) {
     u32 r[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const u64 i = base + (u64)u * 64 + lane;
+      const u64 i = lo + (u64)(j + u * kWaves) * 64 + lane;
       r[u] = (i < hi) ? rot_test<W>(planes, cap, i, g, wa, wb, needx, needz) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const u64 i = base + (u64)u * 64 + lane;
-      bool anti = false;
-      if (r[u]) anti = !is_dead(coef[i]);
-      const u64 m = __ballot(anti);
-      const u64 wbase = base + (u64)u * 64;
-      if (wbase < hi) {
-        if (lane == 0) masks[wbase >> 6] = m;
+      const u64 m = __ballot(r[u] != 0u);
+      const u32 ju = j + u * kWaves;
+      if (ju < nw) {
+        if (lane == 0) masks[(lo >> 6) + ju] = m;
         count += (u64)__popcll(m);
       }
     }
   }
-  if (lane == 0) cnt[blockIdx.x] = count;
+  const u64 tot = block_sum_all(count, lds);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
 // ---- k_rot_gather ---------------------------------------------------------------
+// Copies every anticommuting entry into the worklist in arena order; records
+// of dead entries are kept (flags = 2) so the ranks match the masks.  Block 0
+// also publishes the worklist length.
 template <int W>
-__global__ __launch_bounds__(64) void k_rot_gather(const u64* __restrict__ planes,
-                                                   const double* __restrict__ coef, u64 cap,
-                                                   const DevState* __restrict__ st, u32 in_slot,
-                                                   const u64* __restrict__ masks,
-                                                   const u64* __restrict__ off,
-                                                   WLRec<W>* __restrict__ wl, u32 G) {
+__global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ planes,
+                                                       const double* __restrict__ coef, u64 cap,
+                                                       DevState* st, u32 in_slot,
+                                                       const u64* __restrict__ masks,
+                                                       const u64* __restrict__ cnt,
+                                                       WLRec<W>* __restrict__ wl, u32 G) {
+  __shared__ u64 lds[kWaves];
+  __shared__ u64 lm[kChunkMaxWords];
+  __shared__ u32 loff[kChunkMaxWords];
+  __shared__ u32 lt[kBlock];
   if (st->overflow) return;
-  const u32 lane = threadIdx.x;
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
   u64 lo, hi;
-  wave_chunk(n, G, blockIdx.x, lo, hi);
-  u64 o = off[blockIdx.x];
-  for (u64 base = lo; base < hi; base += 64) {
-    const u64 m = masks[base >> 6];
-    if (m == 0) continue;
+  block_chunk(n, G, blockIdx.x, lo, hi);
+  const u32 nw = (u32)((hi - lo + 63) / 64);
+  const u64 base_off = chunk_offset(cnt, blockIdx.x, lds);
+  if (blockIdx.x == 0) {
+    const u64 tot = chunk_offset(cnt, G, lds);
+    if (threadIdx.x == 0) st->n_wl = tot;
+  }
+  chunk_mask_scan(masks, lo >> 6, nw, lm, loff, lt);
+  for (u32 j = wave; j < nw; j += kWaves) {
+    const u64 m = lm[j];
     if ((m >> lane) & 1ull) {
-      const u64 i = base + lane;
-      WLRec<W>& r = wl[o + (u64)__popcll(m & lanes_below(lane))];
+      const u64 i = lo + (u64)j * 64 + lane;
+      WLRec<W>& r = wl[base_off + loff[j] + (u64)__popcll(m & lanes_below(lane))];
       u64 k[2 * W];
       load_key<W>(planes, cap, i, k);
+      const double c = coef[i];
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) r.k[p] = k[p];
-      r.c = coef[i];
+      r.c = c;
       r.idx = (u32)i;
-      r.flags = 0u;
+      r.flags = is_dead(c) ? 2u : 0u;
     }
-    o += (u64)__popcll(m);
   }
 }
 
@@ -256,6 +323,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
   u64 kills = 0, pairs = 0;
   const u64 stride = (u64)gridDim.x * kBlock;
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n_wl; i += stride) {
+    if (wl[i].flags == 2u) continue;  // dead entry
     u64 k[2 * W];
 #pragma unroll
     for (int p = 0; p < 2 * W; ++p) k[p] = wl[i].k[p];
@@ -317,22 +385,24 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
 
 // ---- k_emit_count -----------------------------------------------------------------
 template <int W>
-__global__ __launch_bounds__(64) void k_emit_count(double* __restrict__ coef, u64 cap,
-                                                   const WLRec<W>* __restrict__ wl,
-                                                   const DevState* __restrict__ st, u32 in_slot,
-                                                   RotSpec g, TruncSpec t, u64* __restrict__ masks,
-                                                   u64* __restrict__ cnt, u64* __restrict__ kills_out,
-                                                   u32 G) {
+__global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef, u64 cap,
+                                                       const WLRec<W>* __restrict__ wl,
+                                                       const DevState* __restrict__ st, u32 in_slot,
+                                                       RotSpec g, TruncSpec t, u64* __restrict__ masks,
+                                                       u64* __restrict__ cnt, u64* __restrict__ kills_out,
+                                                       u32 G) {
+  __shared__ u64 lds[kWaves];
   if (st->overflow) return;
   const u64 n_entries = st->slot[in_slot].n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
-  const u32 lane = threadIdx.x;
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   u64 lo, hi;
-  wave_chunk(n_wl, G, blockIdx.x, lo, hi);
+  block_chunk(n_wl, G, blockIdx.x, lo, hi);
+  const u32 nw = (u32)((hi - lo + 63) / 64);
   u64 count = 0, kills = 0;
-  for (u64 base = lo; base < hi; base += 64) {
-    const u64 w = base + lane;
+  for (u32 j = wave; j < nw; j += kWaves) {
+    const u64 w = lo + (u64)j * 64 + lane;
     bool keep = false;
     if (w < hi && wl[w].flags == 0u) {
       u64 k[2 * W];
@@ -353,86 +423,101 @@ __global__ __launch_bounds__(64) void k_emit_count(double* __restrict__ coef, u6
       kills += (u64)dp;
     }
     const u64 m = __ballot(keep);
-    if (lane == 0) masks[base >> 6] = m;
+    if (lane == 0) masks[(lo >> 6) + j] = m;
     count += (u64)__popcll(m);
   }
-  for (int off = 32; off > 0; off >>= 1) kills += __shfl_down(kills, off, 64);
-  if (lane == 0) {
-    cnt[blockIdx.x] = count;
-    kills_out[blockIdx.x] = kills;
+  const u64 tc = block_sum_all(count, lds);
+  const u64 tk = block_sum_all(kills, lds);
+  if (threadIdx.x == 0) {
+    cnt[blockIdx.x] = tc;
+    kills_out[blockIdx.x] = tk;
   }
 }
 
 // ---- k_emit_write -----------------------------------------------------------------
+// Appends the surviving children; block 0 also publishes the op's new counts
+// and statistics.
 template <int W>
-__global__ __launch_bounds__(64) void k_emit_write(u64* __restrict__ planes, double* __restrict__ coef,
-                                                   u64 cap, const WLRec<W>* __restrict__ wl,
-                                                   const DevState* __restrict__ st, u32 in_slot,
-                                                   RotSpec g, const u64* __restrict__ masks,
-                                                   const u64* __restrict__ off, u32 G) {
+__global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes, double* __restrict__ coef,
+                                                       u64 cap, const WLRec<W>* __restrict__ wl,
+                                                       DevState* st, u32 in_slot, u32 out_slot,
+                                                       RotSpec g, const u64* __restrict__ masks,
+                                                       const u64* __restrict__ cnt,
+                                                       const u64* __restrict__ ekills,
+                                                       const u64* __restrict__ pkills,
+                                                       const u64* __restrict__ ppairs, u32 G, u32 Gp) {
+  __shared__ u64 lds[kWaves];
+  __shared__ u64 lm[kChunkMaxWords];
+  __shared__ u32 loff[kChunkMaxWords];
+  __shared__ u32 lt[kBlock];
   if (st->overflow) return;
-  const u64 n_entries = st->slot[in_slot].n_entries;
+  const CountSlot in = st->slot[in_slot];
+  const u64 n_entries = in.n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
-  const u32 lane = threadIdx.x;
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   u64 lo, hi;
-  wave_chunk(n_wl, G, blockIdx.x, lo, hi);
-  u64 o = n_entries + off[blockIdx.x];
-  for (u64 base = lo; base < hi; base += 64) {
-    const u64 m = masks[base >> 6];
-    if (m == 0) continue;
+  block_chunk(n_wl, G, blockIdx.x, lo, hi);
+  const u32 nw = (u32)((hi - lo + 63) / 64);
+  const u64 base_off = n_entries + chunk_offset(cnt, blockIdx.x, lds);
+  if (blockIdx.x == 0) {
+    const u64 appended = chunk_offset(cnt, G, lds);
+    u64 s = 0, sp = 0;
+    for (u32 j = threadIdx.x; j < G; j += kBlock) s += ekills[j];
+    for (u32 j = threadIdx.x; j < Gp; j += kBlock) {
+      s += pkills[j];
+      sp += ppairs[j];
+    }
+    const u64 kills = block_sum_all(s, lds);
+    const u64 pairs = block_sum_all(sp, lds);
+    if (threadIdx.x == 0) {
+      st->slot[out_slot].n_entries = n_entries + appended;
+      st->slot[out_slot].n_dead = in.n_dead + kills;
+      st->term_updates += n_entries - in.n_dead;
+      st->gates += 1;
+      st->rotations += 1;
+      st->scanned += n_entries;
+      st->scan_words += n_entries * g.test_words;
+      st->wl_total += n_wl;
+      st->pairs_total += pairs;
+      st->children_total += appended;
+      st->kills_total += kills;
+    }
+  }
+  chunk_mask_scan(masks, lo >> 6, nw, lm, loff, lt);
+  for (u32 j = wave; j < nw; j += kWaves) {
+    const u64 m = lm[j];
     if ((m >> lane) & 1ull) {
-      const u64 w = base + lane;
+      const u64 w = lo + (u64)j * 64 + lane;
       u64 k[2 * W];
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) k[p] = wl[w].k[p];
       const double eps = rot_eps<W>(k, g);
       const double cc = __dmul_rn(wl[w].c, eps * g.sn);
       rot_flip<W>(k, g);
-      const u64 dst = o + (u64)__popcll(m & lanes_below(lane));
+      const u64 dst = base_off + loff[j] + (u64)__popcll(m & lanes_below(lane));
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) planes[(u64)p * cap + dst] = k[p];
       coef[dst] = cc;
     }
-    o += (u64)__popcll(m);
   }
 }
 
-// ---- k_finish: single-block scan / reduction / bookkeeping ------------------------
-enum FinishMode : u32 { F_WORKLIST = 0, F_EMIT = 1, F_MAP = 2, F_COMPACT = 3, F_OVERLAP = 4 };
+// ---- k_finish: single-block reduction / bookkeeping ---------------------------
+enum FinishMode : u32 { F_MAP = 0, F_OVERLAP = 1 };
 struct FinishArgs {
   u32 mode;
   u32 in_slot, out_slot;
-  u32 G;               // entries of `cnt` to scan (0: none)
-  u32 Ga;              // entries of aux0 / aux1 to sum
-  u32 Gb;              // entries of aux2 / aux3 to sum
+  u32 Ga;              // partial-sum entries to reduce
   u32 n_ops;           // map program length (F_MAP)
-  u32 test_words;      // rotation test words (F_EMIT)
-  u64* cnt;            // in: counts, out: exclusive offsets
-  const u64* aux0;     // kills (emit count / map)
-  const u64* aux1;     // updates (map)
-  const u64* aux2;     // kills (pair)
-  const u64* aux3;     // pairs (pair)
+  const u64* aux0;     // kills (F_MAP)
+  const u64* aux1;     // term updates (F_MAP)
   const double* dpart; // overlap partials (F_OVERLAP)
 };
 
-__device__ __forceinline__ u64 block_reduce_1024(u64 v, u64* lds) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) lds[wave] = v;
-  __syncthreads();
-  u64 t = 0;
-  for (u32 w = 0; w < blockDim.x / 64; ++w) t += lds[w];
-  __syncthreads();
-  return t;  // valid in every thread
-}
-
 __global__ __launch_bounds__(kScanThreads) void k_finish(DevState* st, FinishArgs a) {
   __shared__ u64 lds[kScanThreads / 64];
-  __shared__ u64 lds_scan[kScanThreads];
   __shared__ double lds_d[kScanThreads];
-  if (a.mode != F_OVERLAP && a.mode != F_COMPACT && st->overflow) return;
   const u32 tid = threadIdx.x;
   if (a.mode == F_OVERLAP) {
     // ordered sum of per-block partials (deterministic)
@@ -448,72 +533,37 @@ __global__ __launch_bounds__(kScanThreads) void k_finish(DevState* st, FinishArg
     }
     return;
   }
-  // exclusive scan of cnt[0..G)
-  u64 total = 0;
-  if (a.G) {
-    const u32 per = (a.G + kScanThreads - 1) / kScanThreads;
-    const u32 j0 = tid * per;
-    u64 local = 0;
-    for (u32 j = j0; j < j0 + per && j < a.G; ++j) local += a.cnt[j];
-    lds_scan[tid] = local;
-    __syncthreads();
-    // Hillis-Steele inclusive scan over 1024 partial sums
-    for (u32 d = 1; d < kScanThreads; d <<= 1) {
-      const u64 add = tid >= d ? lds_scan[tid - d] : 0ull;
-      __syncthreads();
-      lds_scan[tid] += add;
-      __syncthreads();
-    }
-    total = lds_scan[kScanThreads - 1];
-    u64 run = lds_scan[tid] - local;
-    for (u32 j = j0; j < j0 + per && j < a.G; ++j) {
-      const u64 c = a.cnt[j];
-      a.cnt[j] = run;
-      run += c;
-    }
-  }
-  u64 s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  if (st->overflow) return;
+  u64 s0 = 0, s1 = 0;
   for (u32 j = tid; j < a.Ga; j += kScanThreads) {
-    if (a.aux0) s0 += a.aux0[j];
-    if (a.aux1) s1 += a.aux1[j];
+    s0 += a.aux0[j];
+    s1 += a.aux1[j];
   }
-  for (u32 j = tid; j < a.Gb; j += kScanThreads) {
-    if (a.aux2) s2 += a.aux2[j];
-    if (a.aux3) s3 += a.aux3[j];
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_down(s0, o, 64);
+    s1 += __shfl_down(s1, o, 64);
   }
-  s0 = block_reduce_1024(s0, lds);
-  s1 = block_reduce_1024(s1, lds);
-  s2 = block_reduce_1024(s2, lds);
-  s3 = block_reduce_1024(s3, lds);
+  const u32 lane = tid & 63u, wave = tid >> 6;
+  __shared__ u64 lds1[kScanThreads / 64];
+  if (lane == 0) {
+    lds[wave] = s0;
+    lds1[wave] = s1;
+  }
+  __syncthreads();
   if (tid != 0) return;
-  const CountSlot in = st->slot[a.in_slot];
-  if (a.mode == F_WORKLIST) {
-    st->n_wl = total;
-  } else if (a.mode == F_EMIT) {
-    const u64 kills = s0 + s2;
-    st->slot[a.out_slot].n_entries = in.n_entries + total;
-    st->slot[a.out_slot].n_dead = in.n_dead + kills;
-    st->term_updates += in.n_entries - in.n_dead;
-    st->gates += 1;
-    st->rotations += 1;
-    st->scanned += in.n_entries;
-    st->scan_words += in.n_entries * a.test_words;
-    st->wl_total += st->n_wl;
-    st->pairs_total += s3;
-    st->children_total += total;
-    st->kills_total += kills;
-  } else if (a.mode == F_MAP) {
-    st->slot[a.out_slot].n_entries = in.n_entries;
-    st->slot[a.out_slot].n_dead = in.n_dead + s0;
-    st->term_updates += s1;
-    st->gates += a.n_ops;
-    st->kills_total += s0;
-    st->map_entries += in.n_entries;
-    st->map_launches += 1;
-  } else if (a.mode == F_COMPACT) {
-    st->slot[a.out_slot].n_entries = total;
-    st->slot[a.out_slot].n_dead = 0;
+  u64 kills = 0, upd = 0;
+  for (u32 w = 0; w < kScanThreads / 64; ++w) {
+    kills += lds[w];
+    upd += lds1[w];
   }
+  const CountSlot in = st->slot[a.in_slot];
+  st->slot[a.out_slot].n_entries = in.n_entries;
+  st->slot[a.out_slot].n_dead = in.n_dead + kills;
+  st->term_updates += upd;
+  st->gates += a.n_ops;
+  st->kills_total += kills;
+  st->map_entries += in.n_entries;
+  st->map_launches += 1;
 }
 
 // ---- k_map: Clifford / noise / truncation program over every entry ------------
@@ -643,47 +693,62 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ plan
 }
 
 // ---- dead-entry compaction into the other arena --------------------------------
-__global__ __launch_bounds__(64) void k_compact_count(const double* __restrict__ coef,
-                                                      const DevState* __restrict__ st, u32 in_slot,
-                                                      u64* __restrict__ masks, u64* __restrict__ cnt,
-                                                      u32 G) {
-  const u32 lane = threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_compact_count(const double* __restrict__ coef,
+                                                          const DevState* __restrict__ st, u32 in_slot,
+                                                          u64* __restrict__ masks, u64* __restrict__ cnt,
+                                                          u32 G) {
+  __shared__ u64 lds[kWaves];
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
   u64 lo, hi;
-  wave_chunk(n, G, blockIdx.x, lo, hi);
+  block_chunk(n, G, blockIdx.x, lo, hi);
+  const u32 nw = (u32)((hi - lo + 63) / 64);
   u64 count = 0;
-  for (u64 base = lo; base < hi; base += 64) {
-    const u64 i = base + lane;
+  for (u32 j = wave; j < nw; j += kWaves) {
+    const u64 i = lo + (u64)j * 64 + lane;
     const bool live = i < hi && !is_dead(coef[i]);
     const u64 m = __ballot(live);
-    if (lane == 0) masks[base >> 6] = m;
+    if (lane == 0) masks[(lo >> 6) + j] = m;
     count += (u64)__popcll(m);
   }
-  if (lane == 0) cnt[blockIdx.x] = count;
+  const u64 tot = block_sum_all(count, lds);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
 template <int W>
-__global__ __launch_bounds__(64) void k_compact_write(const u64* __restrict__ sp,
-                                                      const double* __restrict__ sc,
-                                                      u64* __restrict__ dp, double* __restrict__ dc,
-                                                      u64 cap, const DevState* __restrict__ st,
-                                                      u32 in_slot, const u64* __restrict__ masks,
-                                                      const u64* __restrict__ off, u32 G) {
-  const u32 lane = threadIdx.x;
+__global__ __launch_bounds__(kBlock) void k_compact_write(const u64* __restrict__ sp,
+                                                          const double* __restrict__ sc,
+                                                          u64* __restrict__ dp, double* __restrict__ dc,
+                                                          u64 cap, DevState* st, u32 in_slot, u32 out_slot,
+                                                          const u64* __restrict__ masks,
+                                                          const u64* __restrict__ cnt, u32 G) {
+  __shared__ u64 lds[kWaves];
+  __shared__ u64 lm[kChunkMaxWords];
+  __shared__ u32 loff[kChunkMaxWords];
+  __shared__ u32 lt[kBlock];
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
   u64 lo, hi;
-  wave_chunk(n, G, blockIdx.x, lo, hi);
-  u64 o = off[blockIdx.x];
-  for (u64 base = lo; base < hi; base += 64) {
-    const u64 m = masks[base >> 6];
+  block_chunk(n, G, blockIdx.x, lo, hi);
+  const u32 nw = (u32)((hi - lo + 63) / 64);
+  const u64 base_off = chunk_offset(cnt, blockIdx.x, lds);
+  if (blockIdx.x == 0) {
+    const u64 tot = chunk_offset(cnt, G, lds);
+    if (threadIdx.x == 0) {
+      st->slot[out_slot].n_entries = tot;
+      st->slot[out_slot].n_dead = 0;
+    }
+  }
+  chunk_mask_scan(masks, lo >> 6, nw, lm, loff, lt);
+  for (u32 j = wave; j < nw; j += kWaves) {
+    const u64 m = lm[j];
     if ((m >> lane) & 1ull) {
-      const u64 i = base + lane;
-      const u64 dst = o + (u64)__popcll(m & lanes_below(lane));
+      const u64 i = lo + (u64)j * 64 + lane;
+      const u64 dst = base_off + loff[j] + (u64)__popcll(m & lanes_below(lane));
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) dp[(u64)p * cap + dst] = sp[(u64)p * cap + i];
       dc[dst] = sc[i];
     }
-    o += (u64)__popcll(m);
   }
 }
 
