@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ pl
       }
     }
   }
-  const u64 tot = block_sum_all(count, lds);
+  const u64 tot = block_sum_all(lane == 0 ? count : 0ull, lds);  // count is wave-uniform
   if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
     if (lane == 0) masks[(lo >> 6) + j] = m;
     count += (u64)__popcll(m);
   }
-  const u64 tc = block_sum_all(count, lds);
+  const u64 tc = block_sum_all(lane == 0 ? count : 0ull, lds);  // count is wave-uniform
   const u64 tk = block_sum_all(kills, lds);
   if (threadIdx.x == 0) {
     cnt[blockIdx.x] = tc;
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_count(const double* __restri
     if (lane == 0) masks[(lo >> 6) + j] = m;
     count += (u64)__popcll(m);
   }
-  const u64 tot = block_sum_all(count, lds);
+  const u64 tot = block_sum_all(lane == 0 ? count : 0ull, lds);  // count is wave-uniform
   if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
 }
 
