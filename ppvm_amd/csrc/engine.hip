@@ -275,15 +275,15 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
 enum Phase { PH_COUNT = 0, PH_GATHER = 1, PH_PAIR = 2, PH_ECOUNT = 3, PH_EWRITE = 4, PH_MAP = 5,
              PH_FINISH = 6, PH_OTHER = 7 };
 
-// Block chunks for a pass over at most n_bound items in an arena of capacity
-// cap: at most kChunkMaxWords mask words per chunk, ~32K items per chunk.
-u32 chunks_for(const ppvm_sum* s) {
-  const u64 per_max = (u64)kChunkMaxWords * 64;
-  u64 g = std::max<u64>((s->cap + per_max - 1) / per_max, (s->n_bound + 32767) / 32768);
+// Blocks to launch for a chunked pass over at most `bound` items (the kernels
+// derive the exact chunk count from the actual n; extra blocks exit at once).
+u32 chunk_blocks(u64 bound, u32 target) {
+  u64 g = (bound + target - 1) / target;
   if (g < 1) g = 1;
   if (g > (u64)kMaxParts) g = kMaxParts;
   return (u32)g;
 }
+constexpr int kFixedGrid = 2048;  // grid-stride kernels with per-block partials
 
 int launch_finish(ppvm_sum* s, const FinishArgs& a) {
   int rc;
@@ -296,31 +296,32 @@ template <int W>
 int launch_rot(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
   int rc;
-  const u32 G = chunks_for(s);
+  const u32 G = chunk_blocks(s->n_bound, kArenaChunk);
+  const u32 Gl = chunk_blocks(s->n_bound, kListChunk);
   WLRec<W>* wl = reinterpret_cast<WLRec<W>*>(s->wl);
   u64* planes = s->planes[s->cur];
   double* coef = s->coef[s->cur];
   if ((rc = prof_begin(s, PH_COUNT))) return rc;
   hipLaunchKernelGGL(k_rot_count<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, s->cap, s->st, in,
-                     it.rot, s->masks, s->part(0), G);
+                     it.rot, s->masks, s->part(0));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_GATHER))) return rc;
   hipLaunchKernelGGL(k_rot_gather<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, coef, s->cap, s->st, in,
-                     s->masks, s->part(0), wl, G);
+                     s->masks, s->part(0), wl);
   if ((rc = prof_end(s))) return rc;
   const u32 tg = next_tab_gen(s);
-  const u32 Gp = (u32)grid_for(s->n_bound, kBlock * 2, kMaxParts);
+  const u32 Gp = (u32)grid_for(s->n_bound, kBlock * 2, kFixedGrid);
   if ((rc = prof_begin(s, PH_PAIR))) return rc;
   hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
                      it.rot, s->table, s->tcap, tg, s->trunc_spec(), it.seq + 1, s->part(1), s->part(2));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_ECOUNT))) return rc;
-  hipLaunchKernelGGL(k_emit_count<W>, dim3(G), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
-                     it.rot, s->trunc_spec(), s->masks, s->part(3), s->part(4), G);
+  hipLaunchKernelGGL(k_emit_count<W>, dim3(Gl), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
+                     it.rot, s->trunc_spec(), s->masks, s->part(3), s->part(4));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_EWRITE))) return rc;
-  hipLaunchKernelGGL(k_emit_write<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, coef, s->cap, wl, s->st,
-                     in, out, it.rot, s->masks, s->part(3), s->part(4), s->part(1), s->part(2), G, Gp);
+  hipLaunchKernelGGL(k_emit_write<W>, dim3(Gl), dim3(kBlock), 0, s->stream, planes, coef, s->cap, wl, s->st,
+                     in, out, it.rot, s->masks, s->part(3), s->part(4), s->part(1), s->part(2), Gp);
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
   return PPVM_OK;
@@ -330,7 +331,7 @@ template <int W>
 int launch_map(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
   int rc;
-  const u32 Gm = (u32)grid_for(s->n_bound, kBlock * 4, kMaxParts);
+  const u32 Gm = (u32)grid_for(s->n_bound, kBlock * 4, kFixedGrid);
   if ((rc = prof_begin(s, PH_MAP))) return rc;
   hipLaunchKernelGGL(k_map<W>, dim3(Gm), dim3(kBlock), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
                      s->cap, s->st, in, it.map, s->trunc_spec(), s->part(0), s->part(1));
@@ -369,15 +370,15 @@ template <int W>
 int launch_compact(ppvm_sum* s) {
   int rc;
   const u32 in = (u32)(s->op_seq & 1), out = in ^ 1u;
-  const u32 G = chunks_for(s);
+  const u32 G = chunk_blocks(s->n_bound, kArenaChunk);
   if ((rc = prof_begin(s, PH_OTHER))) return rc;
   hipLaunchKernelGGL(k_compact_count, dim3(G), dim3(kBlock), 0, s->stream, s->coef[s->cur], s->st, in,
-                     s->masks, s->part(0), G);
+                     s->masks, s->part(0));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_OTHER))) return rc;
   hipLaunchKernelGGL(k_compact_write<W>, dim3(G), dim3(kBlock), 0, s->stream, s->planes[s->cur],
                      s->coef[s->cur], s->planes[s->cur ^ 1], s->coef[s->cur ^ 1], s->cap, s->st, in, out,
-                     s->masks, s->part(0), G);
+                     s->masks, s->part(0));
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
   s->op_seq += 1;
@@ -460,7 +461,7 @@ int sync_check(ppvm_sum* s) {
   const u64 live = c.n_entries - c.n_dead;
   s->hs.peak_live = std::max(s->hs.peak_live, live);
   // compaction when dead entries dominate the scans
-  if (c.n_dead > 4096 && c.n_dead * 4 > c.n_entries) {
+  if (c.n_dead > 4096 && c.n_dead * 8 > c.n_entries) {
     if ((rc = compact_w(s))) {
       s->broken = true;
       return rc;

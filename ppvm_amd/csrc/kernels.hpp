@@ -125,13 +125,25 @@ __device__ __forceinline__ void load_key(const u64* __restrict__ planes, u64 cap
 
 // ---- chunked passes ------------------------------------------------------------
 // A pass over n items is split into G contiguous block chunks whose sizes are
-// multiples of 256 (4 mask words) and at most kChunkMaxWords mask words; the
-// host picks G >= capacity / (64 kChunkMaxWords).  A count kernel writes each
+// multiples of 256 (4 mask words) and at most kChunkMaxWords mask words; G is
+// derived on the device from n and a target chunk size.  A count kernel writes each
 // chunk's ballot masks and count; the matching write kernel derives its
 // chunk's offset from the counts of the chunks before it and ranks the items
 // inside the chunk with a scan of the mask popcounts in LDS.
 constexpr int kChunkMaxWords = 1024;
+constexpr u32 kArenaChunk = 16384;  // target items per chunk, arena passes
+constexpr u32 kListChunk = 2048;    // target items per chunk, worklist passes
 
+// Effective chunk count for n items with a target chunk size `target`
+// (computed on the device from the actual n, so the launch may carry extra
+// blocks; those exit at once).  Chunk sizes are multiples of 256 and at most
+// kChunkMaxWords * 64 items, which holds for n <= kMaxParts * kChunkMaxWords * 64.
+__device__ __forceinline__ u32 chunk_count(u64 n, u32 target) {
+  u64 g = (n + target - 1) / target;
+  if (g < 1) g = 1;
+  if (g > (u64)kMaxParts) g = kMaxParts;
+  return (u32)g;
+}
 __device__ __forceinline__ void block_chunk(u64 n, u32 G, u32 b, u64& lo, u64& hi) {
   u64 per = (n + G - 1) / G;
   per = (per + 255) & ~255ull;
@@ -224,11 +236,13 @@ template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ planes, u64 cap,
                                                       const DevState* __restrict__ st, u32 in_slot,
                                                       RotSpec g, u64* __restrict__ masks,
-                                                      u64* __restrict__ cnt, u32 G) {
+                                                      u64* __restrict__ cnt) {
   __shared__ u64 lds[kWaves];
   if (st->overflow) return;
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
+  const u32 G = chunk_count(n, kArenaChunk);
+  if (blockIdx.x >= G) return;
   u64 lo, hi;
   block_chunk(n, G, blockIdx.x, lo, hi);
   const u32 nw = (u32)((hi - lo + 63) / 64);
@@ -267,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ p
                                                        DevState* st, u32 in_slot,
                                                        const u64* __restrict__ masks,
                                                        const u64* __restrict__ cnt,
-                                                       WLRec<W>* __restrict__ wl, u32 G) {
+                                                       WLRec<W>* __restrict__ wl) {
   __shared__ u64 lds[kWaves];
   __shared__ u64 lm[kChunkMaxWords];
   __shared__ u32 loff[kChunkMaxWords];
@@ -275,6 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ p
   if (st->overflow) return;
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
+  const u32 G = chunk_count(n, kArenaChunk);
+  if (blockIdx.x >= G) return;
   u64 lo, hi;
   block_chunk(n, G, blockIdx.x, lo, hi);
   const u32 nw = (u32)((hi - lo + 63) / 64);
@@ -389,13 +405,14 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
                                                        const WLRec<W>* __restrict__ wl,
                                                        const DevState* __restrict__ st, u32 in_slot,
                                                        RotSpec g, TruncSpec t, u64* __restrict__ masks,
-                                                       u64* __restrict__ cnt, u64* __restrict__ kills_out,
-                                                       u32 G) {
+                                                       u64* __restrict__ cnt, u64* __restrict__ kills_out) {
   __shared__ u64 lds[kWaves];
   if (st->overflow) return;
   const u64 n_entries = st->slot[in_slot].n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
+  const u32 G = chunk_count(n_wl, kListChunk);
+  if (blockIdx.x >= G) return;
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   u64 lo, hi;
   block_chunk(n_wl, G, blockIdx.x, lo, hi);
@@ -445,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
                                                        const u64* __restrict__ cnt,
                                                        const u64* __restrict__ ekills,
                                                        const u64* __restrict__ pkills,
-                                                       const u64* __restrict__ ppairs, u32 G, u32 Gp) {
+                                                       const u64* __restrict__ ppairs, u32 Gp) {
   __shared__ u64 lds[kWaves];
   __shared__ u64 lm[kChunkMaxWords];
   __shared__ u32 loff[kChunkMaxWords];
@@ -455,6 +472,8 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
   const u64 n_entries = in.n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
+  const u32 G = chunk_count(n_wl, kListChunk);
+  if (blockIdx.x >= G) return;
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   u64 lo, hi;
   block_chunk(n_wl, G, blockIdx.x, lo, hi);
@@ -695,11 +714,12 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ plan
 // ---- dead-entry compaction into the other arena --------------------------------
 __global__ __launch_bounds__(kBlock) void k_compact_count(const double* __restrict__ coef,
                                                           const DevState* __restrict__ st, u32 in_slot,
-                                                          u64* __restrict__ masks, u64* __restrict__ cnt,
-                                                          u32 G) {
+                                                          u64* __restrict__ masks, u64* __restrict__ cnt) {
   __shared__ u64 lds[kWaves];
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
+  const u32 G = chunk_count(n, kArenaChunk);
+  if (blockIdx.x >= G) return;
   u64 lo, hi;
   block_chunk(n, G, blockIdx.x, lo, hi);
   const u32 nw = (u32)((hi - lo + 63) / 64);
@@ -721,13 +741,15 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const u64* __restrict_
                                                           u64* __restrict__ dp, double* __restrict__ dc,
                                                           u64 cap, DevState* st, u32 in_slot, u32 out_slot,
                                                           const u64* __restrict__ masks,
-                                                          const u64* __restrict__ cnt, u32 G) {
+                                                          const u64* __restrict__ cnt) {
   __shared__ u64 lds[kWaves];
   __shared__ u64 lm[kChunkMaxWords];
   __shared__ u32 loff[kChunkMaxWords];
   __shared__ u32 lt[kBlock];
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
+  const u32 G = chunk_count(n, kArenaChunk);
+  if (blockIdx.x >= G) return;
   u64 lo, hi;
   block_chunk(n, G, blockIdx.x, lo, hi);
   const u32 nw = (u32)((hi - lo + 63) / 64);
