@@ -37,6 +37,32 @@ struct alignas(16) WLRec {
   u32 flags;  // 1 = merged with its partner by k_rot_pair
 };
 
+// Worklist record load with 16-byte vector loads.
+template <int W>
+__device__ __forceinline__ void load_rec(const WLRec<W>* __restrict__ wl, u64 i, u64 (&k)[2 * W], double& c,
+                                         u32& idx, u32& flags) {
+  const ulonglong2* p = reinterpret_cast<const ulonglong2*>(wl + i);
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    const ulonglong2 v = p[q];
+    k[2 * q] = v.x;
+    k[2 * q + 1] = v.y;
+  }
+  const ulonglong2 t = p[W];
+  c = __longlong_as_double((long long)t.x);
+  idx = (u32)t.y;
+  flags = (u32)(t.y >> 32);
+}
+
+template <int W>
+__device__ __forceinline__ void store_rec(WLRec<W>* __restrict__ wl, u64 i, const u64 (&k)[2 * W], double c,
+                                          u32 idx, u32 flags) {
+  ulonglong2* p = reinterpret_cast<ulonglong2*>(wl + i);
+#pragma unroll
+  for (int q = 0; q < W; ++q) p[q] = make_ulonglong2(k[2 * q], k[2 * q + 1]);
+  p[W] = make_ulonglong2((u64)__double_as_longlong(c), (u64)idx | ((u64)flags << 32));
+}
+
 // Word access with a runtime index, without dynamic register indexing.
 template <int N>
 __device__ __forceinline__ u64 getw(const u64 (&k)[N], u32 i) {
@@ -131,7 +157,7 @@ __device__ __forceinline__ void load_key(const u64* __restrict__ planes, u64 cap
 // chunk's offset from the counts of the chunks before it and ranks the items
 // inside the chunk with a scan of the mask popcounts in LDS.
 constexpr int kChunkMaxWords = 1024;
-constexpr u32 kArenaChunk = 16384;  // target items per chunk, arena passes
+constexpr u32 kArenaChunk = 8192;   // target items per chunk, arena passes
 constexpr u32 kListChunk = 2048;    // target items per chunk, worklist passes
 
 // Effective chunk count for n items with a target chunk size `target`
@@ -304,15 +330,11 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ p
     const u64 m = lm[j];
     if ((m >> lane) & 1ull) {
       const u64 i = lo + (u64)j * 64 + lane;
-      WLRec<W>& r = wl[base_off + loff[j] + (u64)__popcll(m & lanes_below(lane))];
       u64 k[2 * W];
       load_key<W>(planes, cap, i, k);
       const double c = coef[i];
-#pragma unroll
-      for (int p = 0; p < 2 * W; ++p) r.k[p] = k[p];
-      r.c = c;
-      r.idx = (u32)i;
-      r.flags = is_dead(c) ? 2u : 0u;
+      store_rec<W>(wl, base_off + loff[j] + (u64)__popcll(m & lanes_below(lane)), k, c, (u32)i,
+                   is_dead(c) ? 2u : 0u);
     }
   }
 }
@@ -339,17 +361,20 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
   u64 kills = 0, pairs = 0;
   const u64 stride = (u64)gridDim.x * kBlock;
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n_wl; i += stride) {
-    if (wl[i].flags == 2u) continue;  // dead entry
     u64 k[2 * W];
-#pragma unroll
-    for (int p = 0; p < 2 * W; ++p) k[p] = wl[i].k[p];
+    double ci;
+    u32 idxi, fli;
+    load_rec<W>(wl, i, k, ci, idxi, fli);
+    if (fli == 2u) continue;  // dead entry
     u64 rep[2 * W];
     pair_key<W>(k, g, rep);
     const u64 h = hash_key<W>(rep);
     const u64 fp = h >> 56;
     const u64 mine = ((u64)tgen << 40) | (fp << 32) | i;
     u64 pos = h & tmask;
-    u64 s = ld_relaxed(&table[pos]);
+    // Plain loads may return a stale (older) value; that can only make a slot
+    // look empty, and the CAS then returns the value actually in memory.
+    u64 s = table[pos];
     while (true) {
       if ((u32)(s >> 40) != tgen) {
         const u64 old = cas_u64(&table[pos], s, mine);
@@ -360,8 +385,9 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
       if (((s >> 32) & 0xffull) == fp) {
         const u64 j = s & 0xffffffffull;
         u64 kj[2 * W];
-#pragma unroll
-        for (int p = 0; p < 2 * W; ++p) kj[p] = wl[j].k[p];
+        double cj;
+        u32 idxj, flj;
+        load_rec<W>(wl, j, kj, cj, idxj, flj);
         u64 repj[2 * W];
         pair_key<W>(kj, g, repj);
         bool same = true;
@@ -369,7 +395,6 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
         for (int p = 0; p < 2 * W; ++p) same &= (repj[p] == rep[p]);
         if (same) {
           // second arrival: merge both members
-          const double ci = wl[i].c, cj = wl[j].c;
           const double ei = rot_eps<W>(k, g), ej = rot_eps<W>(kj, g);
           const double ni = __dadd_rn(__dmul_rn(ci, g.cs), __dmul_rn(cj, ej * g.sn));
           const double nj = __dadd_rn(__dmul_rn(cj, g.cs), __dmul_rn(ci, ei * g.sn));
@@ -378,8 +403,8 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
             di = drop_term<W>(k, ni, t);
             dj = drop_term<W>(kj, nj, t);
           }
-          coef[wl[i].idx] = di ? dead_value() : ni;
-          coef[wl[j].idx] = dj ? dead_value() : nj;
+          coef[idxi] = di ? dead_value() : ni;
+          coef[idxj] = dj ? dead_value() : nj;
           wl[i].flags = 1u;
           wl[j].flags = 1u;
           kills += (u64)di + (u64)dj;
@@ -388,7 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
         }
       }
       pos = (pos + 1) & tmask;
-      s = ld_relaxed(&table[pos]);
+      s = table[pos];
     }
   }
   const u64 bk = block_sum_u64(kills, lds_sum);
@@ -421,11 +446,11 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
   for (u32 j = wave; j < nw; j += kWaves) {
     const u64 w = lo + (u64)j * 64 + lane;
     bool keep = false;
-    if (w < hi && wl[w].flags == 0u) {
-      u64 k[2 * W];
-#pragma unroll
-      for (int p = 0; p < 2 * W; ++p) k[p] = wl[w].k[p];
-      const double c = wl[w].c;
+    u64 k[2 * W];
+    double c = 0.0;
+    u32 idx = 0, fl = 1u;
+    if (w < hi) load_rec<W>(wl, w, k, c, idx, fl);
+    if (fl == 0u) {
       const double eps = rot_eps<W>(k, g);
       const double cp = __dmul_rn(c, g.cs);
       const double cc = __dmul_rn(c, eps * g.sn);
@@ -436,7 +461,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
         rot_flip<W>(k, g);
         keep = !drop_term<W>(k, cc, t);
       }
-      coef[wl[w].idx] = dp ? dead_value() : cp;
+      coef[idx] = dp ? dead_value() : cp;
       kills += (u64)dp;
     }
     const u64 m = __ballot(keep);
@@ -509,10 +534,11 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
     if ((m >> lane) & 1ull) {
       const u64 w = lo + (u64)j * 64 + lane;
       u64 k[2 * W];
-#pragma unroll
-      for (int p = 0; p < 2 * W; ++p) k[p] = wl[w].k[p];
+      double c;
+      u32 idx, fl;
+      load_rec<W>(wl, w, k, c, idx, fl);
       const double eps = rot_eps<W>(k, g);
-      const double cc = __dmul_rn(wl[w].c, eps * g.sn);
+      const double cc = __dmul_rn(c, eps * g.sn);
       rot_flip<W>(k, g);
       const u64 dst = base_off + loff[j] + (u64)__popcll(m & lanes_below(lane));
 #pragma unroll
