@@ -206,6 +206,56 @@ int ppvm_get_counters(ppvm_handle h, uint64_t out[16]);
 int ppvm_profile_enable(ppvm_handle h, int enable);
 int ppvm_profile_read(ppvm_handle h, double ms[8], uint64_t launches[8]);
 
+/* ---- batched mode -------------------------------------------------------------
+ * No counterpart in the reference (it propagates one PauliSum per call); this
+ * runs many independent instances, each with the semantics above, in one
+ * persistent kernel.  An instance is an initial term list, a program of
+ * ppvm_op records (PPVM_OP_TRUNCATE included) and its truncation strategy.
+ * Bit 0 of ppvm_op.reserved marks ops after which (len, overlap_with_zero)
+ * is recorded. */
+typedef struct ppvm_batch_instance {
+  uint32_t n_qubits;
+  uint32_t n_terms;            /* initial terms, `+=` semantics                 */
+  const uint64_t* x;           /* n_terms * ceil(n_qubits/64) words             */
+  const uint64_t* z;
+  const double* c;
+  const ppvm_op* ops;
+  uint32_t n_ops;
+  uint32_t reserved;
+  double min_abs_coeff;
+  uint64_t max_pauli_weight;   /* UINT64_MAX: no weight cap                     */
+  uint64_t capacity;           /* arena capacity in terms (0: default)          */
+} ppvm_batch_instance;
+
+typedef struct ppvm_batch_config {
+  uint32_t threads;            /* workgroup size: multiple of 64, <= 1024        */
+  uint32_t blocks;             /* persistent grid (0: min(instances, 2 x CUs))   */
+  uint32_t segment_ops;        /* ops per launch segment (0: whole program)      */
+  uint32_t use_graph;          /* capture the launch sequence in a hipGraph      */
+} ppvm_batch_config;
+
+typedef struct ppvm_batch* ppvm_batch_handle;
+
+const char* ppvm_batch_last_error(void);
+int ppvm_batch_create(const ppvm_batch_instance* inst, size_t n, ppvm_batch_handle* out);
+int ppvm_batch_destroy(ppvm_batch_handle b);
+/* Re-sizes one instance's arenas (e.g. after it overflowed). */
+int ppvm_batch_set_capacity(ppvm_batch_handle b, size_t i, uint64_t capacity);
+/* Runs the instances listed in order[0..n_order) (NULL: all, in index order;
+ * the queue hands them out in this order) from their initial terms; *ms gets
+ * the device time of the run. */
+int ppvm_batch_run(ppvm_batch_handle b, const ppvm_batch_config* cfg, const uint32_t* order,
+                   size_t n_order, double* ms);
+/* Per instance: status (0 done, 1 arena overflow, 2 not finished), final term
+ * count, last recorded overlap, term updates, peak term count.  Arrays of
+ * length n (any may be NULL). */
+int ppvm_batch_results(ppvm_batch_handle b, uint32_t* status, uint64_t* n_final,
+                       double* overlap_last, uint64_t* term_updates, uint64_t* peak);
+int ppvm_batch_marks(ppvm_batch_handle b, size_t i, uint64_t* lens, double* overlaps, size_t cap,
+                     size_t* n);
+int ppvm_batch_export(ppvm_batch_handle b, size_t i, uint64_t* x, uint64_t* z, double* c, size_t cap,
+                      size_t* n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,298 @@
+"""Batched propagation: many independent instances in one persistent kernel.
+
+``Batch`` wraps the ``ppvm_batch_*`` C ABI.  ``run_hybrid`` is the scheduler
+used for uneven batches: every instance starts in the persistent kernel with
+an arena capped at ``batch_cap`` terms; instances that outgrow it are re-run
+on the full-GPU engine (one handle per instance, several handles on their own
+HIP streams driven from host threads), which is the better fit for large term
+counts.  ``autotune`` picks the workgroup size, graph capture and the cap by
+measuring candidate settings on the batch itself.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+import threading
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _native
+from .ops import OP_DTYPE, pack_strings, words_for
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class BatchInstance(ctypes.Structure):
+    _fields_ = [
+        ("n_qubits", ctypes.c_uint32),
+        ("n_terms", ctypes.c_uint32),
+        ("x", _u64p),
+        ("z", _u64p),
+        ("c", _f64p),
+        ("ops", ctypes.c_void_p),
+        ("n_ops", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("min_abs_coeff", ctypes.c_double),
+        ("max_pauli_weight", ctypes.c_uint64),
+        ("capacity", ctypes.c_uint64),
+    ]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_uint32), ("blocks", ctypes.c_uint32),
+                ("segment_ops", ctypes.c_uint32), ("use_graph", ctypes.c_uint32)]
+
+
+_declared = False
+
+
+def _lib():
+    global _declared
+    L = _native.lib()
+    if not _declared:
+        L.ppvm_batch_last_error.restype = ctypes.c_char_p
+        L.ppvm_batch_create.argtypes = [ctypes.POINTER(BatchInstance), ctypes.c_size_t,
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        L.ppvm_batch_destroy.argtypes = [ctypes.c_void_p]
+        L.ppvm_batch_set_capacity.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64]
+        L.ppvm_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchConfig), _u32p,
+                                     ctypes.c_size_t, _f64p]
+        L.ppvm_batch_results.argtypes = [ctypes.c_void_p, _u32p, _u64p, _f64p, _u64p, _u64p]
+        L.ppvm_batch_marks.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _f64p,
+                                       ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.ppvm_batch_export.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p, _f64p,
+                                        ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        _declared = True
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = _lib().ppvm_batch_last_error().decode("utf-8", "replace")
+        if rc == _native.PPVM_EINVAL:
+            raise ValueError(msg)
+        if rc == _native.PPVM_ENOMEM:
+            raise MemoryError(msg)
+        raise _native.PpvmError(rc, msg)
+
+
+def marked_program(w) -> np.ndarray:
+    """The workload's program with bit 0 of ``reserved`` set at its layer marks."""
+    prog = np.array(w.program, dtype=OP_DTYPE, copy=True)
+    if len(w.layer_marks):
+        prog["reserved"][np.asarray(w.layer_marks, dtype=np.int64)] |= 1
+    return prog
+
+
+class Batch:
+    """Owning wrapper of a ``ppvm_batch_handle`` built from workloads."""
+
+    def __init__(self, workloads, capacity: int = 0):
+        self.workloads = list(workloads)
+        n = len(self.workloads)
+        self._keep = []
+        arr = (BatchInstance * n)()
+        for i, w in enumerate(self.workloads):
+            x, z = pack_strings([s for s, _ in w.observable], w.n_qubits)
+            c = np.asarray([v for _, v in w.observable], dtype=np.float64)
+            prog = marked_program(w)
+            self._keep += [x, z, c, prog]
+            inst = arr[i]
+            inst.n_qubits = w.n_qubits
+            inst.n_terms = len(c)
+            inst.x = x.ctypes.data_as(_u64p)
+            inst.z = z.ctypes.data_as(_u64p)
+            inst.c = c.ctypes.data_as(_f64p)
+            inst.ops = prog.ctypes.data
+            inst.n_ops = len(prog)
+            inst.min_abs_coeff = w.min_abs_coeff
+            inst.max_pauli_weight = (1 << 64) - 1 if w.max_pauli_weight is None else w.max_pauli_weight
+            inst.capacity = capacity
+        h = ctypes.c_void_p()
+        _check(_lib().ppvm_batch_create(arr, n, ctypes.byref(h)))
+        self.h = h
+        self.n = n
+
+    def close(self):
+        if getattr(self, "h", None) is not None:
+            _lib().ppvm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_capacity(self, i: int, cap: int):
+        _check(_lib().ppvm_batch_set_capacity(self.h, i, int(cap)))
+
+    def run(self, threads: int = 256, blocks: int = 0, segment_ops: int = 0, use_graph: bool = False,
+            order=None) -> float:
+        cfg = BatchConfig(threads, blocks, segment_ops, 1 if use_graph else 0)
+        ms = ctypes.c_double()
+        if order is None:
+            _check(_lib().ppvm_batch_run(self.h, ctypes.byref(cfg), None, 0, ctypes.byref(ms)))
+        else:
+            o = np.ascontiguousarray(order, dtype=np.uint32)
+            _check(_lib().ppvm_batch_run(self.h, ctypes.byref(cfg), o.ctypes.data_as(_u32p), len(o),
+                                         ctypes.byref(ms)))
+        return ms.value
+
+    def results(self) -> dict:
+        n = self.n
+        st = np.zeros(n, dtype=np.uint32)
+        nf = np.zeros(n, dtype=np.uint64)
+        ov = np.zeros(n, dtype=np.float64)
+        tu = np.zeros(n, dtype=np.uint64)
+        pk = np.zeros(n, dtype=np.uint64)
+        _check(_lib().ppvm_batch_results(self.h, st.ctypes.data_as(_u32p), nf.ctypes.data_as(_u64p),
+                                         ov.ctypes.data_as(_f64p), tu.ctypes.data_as(_u64p),
+                                         pk.ctypes.data_as(_u64p)))
+        return {"status": st, "n_final": nf, "overlap": ov, "term_updates": tu, "peak": pk}
+
+    def marks(self, i: int):
+        cap = len(self.workloads[i].layer_marks)
+        lens = np.zeros(cap, dtype=np.uint64)
+        ovl = np.zeros(cap, dtype=np.float64)
+        got = ctypes.c_size_t()
+        _check(_lib().ppvm_batch_marks(self.h, i, lens.ctypes.data_as(_u64p), ovl.ctypes.data_as(_f64p),
+                                       cap, ctypes.byref(got)))
+        return lens[:got.value], ovl[:got.value]
+
+    def export(self, i: int):
+        w = self.workloads[i]
+        W = words_for(w.n_qubits)
+        n = int(self.results()["n_final"][i])
+        x = np.zeros((n, W), dtype=np.uint64)
+        z = np.zeros((n, W), dtype=np.uint64)
+        c = np.zeros(n, dtype=np.float64)
+        got = ctypes.c_size_t()
+        _check(_lib().ppvm_batch_export(self.h, i, x.ctypes.data_as(_u64p), z.ctypes.data_as(_u64p),
+                                        c.ctypes.data_as(_f64p), n, ctypes.byref(got)))
+        return x, z, c
+
+
+@dataclass
+class HybridResult:
+    overlaps: list            # per instance: array of overlaps at the layer marks
+    lens: list                # per instance: array of term counts at the layer marks
+    term_updates: np.ndarray
+    peak: np.ndarray
+    on_engine: list           # indices re-run on the full-GPU engine
+    batch_ms: float
+    engine_s: float
+    wall_s: float
+    latency_s: np.ndarray     # per-instance completion time from the start of the run
+
+
+def _engine_run(w):
+    from .paulisum import PauliSum
+    g = PauliSum.new(w.n_qubits, w.observable, min_abs_coeff=w.min_abs_coeff,
+                     max_pauli_weight=w.max_pauli_weight)
+    lens, ovls = [], []
+    prev = 0
+    for m in w.layer_marks:
+        g.apply(w.program[prev:m + 1])
+        prev = m + 1
+        lens.append(len(g))
+        ovls.append(g.overlap_with_zero())
+    if prev < len(w.program):
+        g.apply(w.program[prev:])
+    g.synchronize()
+    tu = g.stats()["term_updates"]
+    pk = g.stats()["peak_live"]
+    return np.array(lens, dtype=np.uint64), np.array(ovls), tu, pk
+
+
+def run_hybrid(batch: Batch, threads: int = 256, use_graph: bool = True, engine_threads: int = 4,
+               order=None) -> HybridResult:
+    """Run every instance; those that overflow the batch arena cap are re-run
+    on the full-GPU engine (``engine_threads`` host threads, one stream each)."""
+    n = batch.n
+    t0 = time.perf_counter()
+    ms = batch.run(threads=threads, use_graph=use_graph, order=order)
+    t_batch = time.perf_counter()
+    res = batch.results()
+    lat = np.full(n, t_batch - t0)
+    overlaps, lens = [None] * n, [None] * n
+    tu = res["term_updates"].astype(np.int64).copy()
+    pk = res["peak"].astype(np.int64).copy()
+    redo = [i for i in range(n) if res["status"][i] != 0]
+    for i in range(n):
+        if res["status"][i] == 0:
+            lens[i], overlaps[i] = batch.marks(i)
+    # large instances: full-GPU engine, largest-program first
+    redo.sort(key=lambda i: -batch.workloads[i].n_gates)
+    lock = threading.Lock()
+    todo = list(redo)
+    errors = []
+
+    def worker():
+        while True:
+            with lock:
+                if not todo:
+                    return
+                i = todo.pop(0)
+            try:
+                ln, ov, t, p = _engine_run(batch.workloads[i])
+            except Exception as e:  # surfaced after join
+                errors.append(e)
+                return
+            with lock:
+                lens[i], overlaps[i], tu[i], pk[i] = ln, ov, t, p
+                lat[i] = time.perf_counter() - t0
+
+    te = time.perf_counter()
+    ths = [threading.Thread(target=worker) for _ in range(max(1, min(engine_threads, len(redo))))]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    if errors:
+        raise errors[0]
+    t_end = time.perf_counter()
+    return HybridResult(overlaps, lens, tu, pk, redo, ms, t_end - te, t_end - t0, lat)
+
+
+def autotune(workloads, caps=(1 << 16, 1 << 18), threads=(256, 512, 1024), graphs=(False, True),
+             cache_path: str | None = None, log=None) -> dict:
+    """Measure candidate (arena cap, workgroup size, graph) settings on the
+    batch and return the fastest, as a dict.  Results are cached in
+    ``cache_path`` (JSON) keyed by a signature of the workloads."""
+    sig = f"{len(workloads)}:{sum(w.n_gates for w in workloads)}:{sum(w.n_qubits for w in workloads)}"
+    if cache_path and os.path.exists(cache_path):
+        try:
+            cached = json.load(open(cache_path))
+            if sig in cached:
+                return cached[sig]
+        except (OSError, ValueError):
+            pass
+    best = None
+    trials = []
+    for cap in caps:
+        b = Batch(workloads, capacity=cap)
+        for th in threads:
+            for gr in graphs:
+                r = run_hybrid(b, threads=th, use_graph=gr)
+                trials.append({"cap": cap, "threads": th, "graph": gr, "wall_s": r.wall_s,
+                               "batch_ms": r.batch_ms, "on_engine": len(r.on_engine)})
+                if log:
+                    log(f"autotune cap={cap} threads={th} graph={gr}: {r.wall_s:.3f}s "
+                        f"(batch {r.batch_ms:.1f} ms, {len(r.on_engine)} on engine)")
+                if best is None or r.wall_s < best["wall_s"]:
+                    best = dict(trials[-1])
+        b.close()
+    out = dict(best, trials=trials, signature=sig)
+    if cache_path:
+        try:
+            cached = json.load(open(cache_path)) if os.path.exists(cache_path) else {}
+        except (OSError, ValueError):
+            cached = {}
+        cached[sig] = out
+        with open(cache_path, "w") as f:
+            json.dump(cached, f, indent=1)
+    return out
