@@ -74,7 +74,7 @@ def test_batch_mixed_programs_and_widths():
         marks = [j for j in range(len(arr)) if arr[j]["code"] == O.TRUNCATE][::7]
         ws.append(WL.Workload(f"mixed{i}", n, 1e-3, 8 if i % 2 else None,
                               [(s, float(c)) for s, c in zip(strings, rng.normal(size=5))], arr, marks))
-    b = Batch(ws)
+    b = Batch(ws, capacity=1 << 14)
     b.run(threads=256)
     _check_batch(b, ws)
 
