@@ -251,6 +251,9 @@ int ppvm_batch_run(ppvm_batch_handle b, const ppvm_batch_config* cfg, const uint
  * length n (any may be NULL). */
 int ppvm_batch_results(ppvm_batch_handle b, uint32_t* status, uint64_t* n_final,
                        double* overlap_last, uint64_t* term_updates, uint64_t* peak);
+/* Per instance: seconds from the start of the last run until the instance
+ * finished (device wall clock). */
+int ppvm_batch_latency(ppvm_batch_handle b, double* seconds);
 int ppvm_batch_marks(ppvm_batch_handle b, size_t i, uint64_t* lens, double* overlaps, size_t cap,
                      size_t* n);
 int ppvm_batch_export(ppvm_batch_handle b, size_t i, uint64_t* x, uint64_t* z, double* c, size_t cap,

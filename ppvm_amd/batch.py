@@ -64,6 +64,7 @@ def _lib():
         L.ppvm_batch_run.argtypes = [ctypes.c_void_p, ctypes.POINTER(BatchConfig), _u32p,
                                      ctypes.c_size_t, _f64p]
         L.ppvm_batch_results.argtypes = [ctypes.c_void_p, _u32p, _u64p, _f64p, _u64p, _u64p]
+        L.ppvm_batch_latency.argtypes = [ctypes.c_void_p, _f64p]
         L.ppvm_batch_marks.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _f64p,
                                        ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.ppvm_batch_export.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p, _f64p,
@@ -154,6 +155,11 @@ class Batch:
                                          pk.ctypes.data_as(_u64p)))
         return {"status": st, "n_final": nf, "overlap": ov, "term_updates": tu, "peak": pk}
 
+    def latency(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=np.float64)
+        _check(_lib().ppvm_batch_latency(self.h, out.ctypes.data_as(_f64p)))
+        return out
+
     def marks(self, i: int):
         cap = len(self.workloads[i].layer_marks)
         lens = np.zeros(cap, dtype=np.uint64)
@@ -209,27 +215,25 @@ def _engine_run(w):
 
 
 def run_hybrid(batch: Batch, threads: int = 256, use_graph: bool = True, engine_threads: int = 4,
-               order=None) -> HybridResult:
-    """Run every instance; those that overflow the batch arena cap are re-run
-    on the full-GPU engine (``engine_threads`` host threads, one stream each)."""
+               order=None, engine_set=None) -> HybridResult:
+    """Run every instance.  Instances in ``engine_set`` start on the full-GPU
+    engine at once (``engine_threads`` host threads, one stream each) while
+    the persistent kernel runs the rest; instances that overflow the batch
+    arena cap are re-run on the engine afterwards."""
     n = batch.n
-    t0 = time.perf_counter()
-    ms = batch.run(threads=threads, use_graph=use_graph, order=order)
-    t_batch = time.perf_counter()
-    res = batch.results()
-    lat = np.full(n, t_batch - t0)
+    engine_set = sorted(set(engine_set or []), key=lambda i: -batch.workloads[i].n_gates)
+    es = set(engine_set)
+    if order is None:
+        order = sorted(range(n), key=lambda i: -batch.workloads[i].n_gates)
+    order = [i for i in order if i not in es]
     overlaps, lens = [None] * n, [None] * n
-    tu = res["term_updates"].astype(np.int64).copy()
-    pk = res["peak"].astype(np.int64).copy()
-    redo = [i for i in range(n) if res["status"][i] != 0]
-    for i in range(n):
-        if res["status"][i] == 0:
-            lens[i], overlaps[i] = batch.marks(i)
-    # large instances: full-GPU engine, largest-program first
-    redo.sort(key=lambda i: -batch.workloads[i].n_gates)
+    tu = np.zeros(n, dtype=np.int64)
+    pk = np.zeros(n, dtype=np.int64)
+    lat = np.zeros(n)
     lock = threading.Lock()
-    todo = list(redo)
+    todo = list(engine_set)
     errors = []
+    t0 = time.perf_counter()
 
     def worker():
         while True:
@@ -246,23 +250,45 @@ def run_hybrid(batch: Batch, threads: int = 256, use_graph: bool = True, engine_
                 lens[i], overlaps[i], tu[i], pk[i] = ln, ov, t, p
                 lat[i] = time.perf_counter() - t0
 
-    te = time.perf_counter()
-    ths = [threading.Thread(target=worker) for _ in range(max(1, min(engine_threads, len(redo))))]
+    ths = [threading.Thread(target=worker) for _ in range(max(1, min(engine_threads, len(todo))))]
     for t in ths:
         t.start()
+    ms = batch.run(threads=threads, use_graph=use_graph, order=order) if order else 0.0
+    res = batch.results()
+    dev_lat = batch.latency()
+    redo = [i for i in order if res["status"][i] != 0]
+    for i in order:
+        if res["status"][i] == 0:
+            lens[i], overlaps[i] = batch.marks(i)
+            tu[i] = int(res["term_updates"][i])
+            pk[i] = int(res["peak"][i])
+            lat[i] = dev_lat[i]
     for t in ths:
         t.join()
+    te = time.perf_counter()
+    if redo:
+        todo.extend(sorted(redo, key=lambda i: -batch.workloads[i].n_gates))
+        ths = [threading.Thread(target=worker) for _ in range(max(1, min(engine_threads, len(redo))))]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
     if errors:
         raise errors[0]
     t_end = time.perf_counter()
-    return HybridResult(overlaps, lens, tu, pk, redo, ms, t_end - te, t_end - t0, lat)
+    return HybridResult(overlaps, lens, tu, pk, list(engine_set) + redo, ms, t_end - te, t_end - t0, lat)
 
 
-def autotune(workloads, caps=(1 << 16, 1 << 18), threads=(256, 512, 1024), graphs=(False, True),
-             cache_path: str | None = None, log=None) -> dict:
-    """Measure candidate (arena cap, workgroup size, graph) settings on the
-    batch and return the fastest, as a dict.  Results are cached in
-    ``cache_path`` (JSON) keyed by a signature of the workloads."""
+def autotune(workloads, caps=(1 << 16, 1 << 18), threads=(256, 1024), graphs=(True,),
+             engine_threads=(4, 8), cache_path: str | None = None, log=None) -> dict:
+    """Measure candidate settings on the batch itself and return the fastest.
+
+    For each arena cap the first trial runs everything through the persistent
+    kernel and learns which instances outgrow the cap; later trials start
+    those on the engine concurrently with the kernel.  Candidates: arena cap,
+    workgroup size, hipGraph capture, engine host threads.  The winner (with
+    its engine set) is cached in ``cache_path`` (JSON) keyed by a signature of
+    the workloads."""
     sig = f"{len(workloads)}:{sum(w.n_gates for w in workloads)}:{sum(w.n_qubits for w in workloads)}"
     if cache_path and os.path.exists(cache_path):
         try:
@@ -275,18 +301,24 @@ def autotune(workloads, caps=(1 << 16, 1 << 18), threads=(256, 512, 1024), graph
     trials = []
     for cap in caps:
         b = Batch(workloads, capacity=cap)
+        first = run_hybrid(b, threads=max(threads), use_graph=False)
+        big = sorted(first.on_engine)
         for th in threads:
             for gr in graphs:
-                r = run_hybrid(b, threads=th, use_graph=gr)
-                trials.append({"cap": cap, "threads": th, "graph": gr, "wall_s": r.wall_s,
-                               "batch_ms": r.batch_ms, "on_engine": len(r.on_engine)})
-                if log:
-                    log(f"autotune cap={cap} threads={th} graph={gr}: {r.wall_s:.3f}s "
-                        f"(batch {r.batch_ms:.1f} ms, {len(r.on_engine)} on engine)")
-                if best is None or r.wall_s < best["wall_s"]:
-                    best = dict(trials[-1])
+                for et in engine_threads:
+                    r = run_hybrid(b, threads=th, use_graph=gr, engine_threads=et, engine_set=big)
+                    t = {"cap": cap, "threads": th, "graph": gr, "engine_threads": et,
+                         "wall_s": r.wall_s, "batch_ms": r.batch_ms, "on_engine": len(r.on_engine),
+                         "engine_set": big}
+                    trials.append(t)
+                    if log:
+                        log(f"autotune cap={cap} threads={th} graph={gr} engine_threads={et}: "
+                            f"{r.wall_s:.3f}s (kernel {r.batch_ms:.0f} ms, {len(r.on_engine)} on engine)")
+                    if best is None or r.wall_s < best["wall_s"]:
+                        best = dict(t)
         b.close()
-    out = dict(best, trials=trials, signature=sig)
+    out = dict(best, trials=[{k: v for k, v in t.items() if k != "engine_set"} for t in trials],
+               signature=sig)
     if cache_path:
         try:
             cached = json.load(open(cache_path)) if os.path.exists(cache_path) else {}

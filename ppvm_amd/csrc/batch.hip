@@ -102,6 +102,7 @@ struct BInst {
   u64 term_updates;
   u64 peak;
   double ovl_last;
+  u64 t_done;     // device wall clock when the instance finished (or stopped)
 };
 
 constexpr int kBMaxThreads = 1024;
@@ -390,14 +391,17 @@ __global__ __launch_bounds__(kBMaxThreads) void k_batch(BInst* insts, const u32*
       Ig->term_updates = I.term_updates;
       Ig->peak = I.peak;
       Ig->ovl_last = I.ovl_last;
+      Ig->t_done = wall_clock64();
     }
     __syncthreads();
   }
 }
 
-// Resets instance state and loads the initial terms (already in arena 0).
-__global__ void k_batch_reset(BInst* insts, u32 n) {
+// Resets instance state (the initial terms are already in arena 0) and stamps
+// the start of the run.
+__global__ void k_batch_reset(BInst* insts, u32 n, u64* t0) {
   const u32 i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *t0 = wall_clock64();
   if (i >= n) return;
   BInst& I = insts[i];
   I.status = IS_READY;
@@ -408,6 +412,7 @@ __global__ void k_batch_reset(BInst* insts, u32 n) {
   I.term_updates = 0;
   I.peak = I.n_init;
   I.ovl_last = 0.0;
+  I.t_done = 0;
 }
 
 __global__ void k_bfill_u32(u32* p, u32 n, u32 v) {
@@ -434,6 +439,7 @@ struct ppvm_batch {
   double* d_factors = nullptr;
   u32* d_order = nullptr;
   u32* d_queue = nullptr;         // one counter per segment launch
+  u64* d_t0 = nullptr;            // device wall clock at the start of a run
   u32 n_queue = 0;
   std::vector<u32> n_qubits;
   std::vector<u32> Wa;
@@ -705,7 +711,8 @@ int ppvm_batch_create(const ppvm_batch_instance* inst, size_t n, ppvm_batch_hand
   if (hipMalloc(&b->d_inst, n * sizeof(BInst)) != hipSuccess ||
       hipMalloc(&b->d_ops, std::max<size_t>(1, ops.size()) * sizeof(BOp)) != hipSuccess ||
       hipMalloc(&b->d_factors, std::max<size_t>(1, facs.size()) * sizeof(double)) != hipSuccess ||
-      hipMalloc(&b->d_order, n * sizeof(u32)) != hipSuccess) {
+      hipMalloc(&b->d_order, n * sizeof(u32)) != hipSuccess ||
+      hipMalloc(&b->d_t0, sizeof(u64)) != hipSuccess) {
     (void)hipGetLastError();
     ppvm_batch_destroy(b);
     return bfail(PPVM_ENOMEM, "batch descriptor allocation failed");
@@ -758,6 +765,7 @@ int ppvm_batch_destroy(ppvm_batch_handle b) {
   if (b->d_factors) (void)hipFree(b->d_factors);
   if (b->d_order) (void)hipFree(b->d_order);
   if (b->d_queue) (void)hipFree(b->d_queue);
+  if (b->d_t0) (void)hipFree(b->d_t0);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return PPVM_OK;
@@ -817,7 +825,7 @@ int ppvm_batch_run(ppvm_batch_handle b, const ppvm_batch_config* cfg, const uint
   BCHK(hipEventCreate(&e0));
   BCHK(hipEventCreate(&e1));
   auto record = [&](hipStream_t st) -> int {
-    hipLaunchKernelGGL(k_batch_reset, dim3((b->n + 255) / 256), dim3(256), 0, st, b->d_inst, b->n);
+    hipLaunchKernelGGL(k_batch_reset, dim3((b->n + 255) / 256), dim3(256), 0, st, b->d_inst, b->n, b->d_t0);
     hipLaunchKernelGGL(k_bfill_u32, dim3((n_seg + 255) / 256), dim3(256), 0, st, b->d_queue, n_seg, 0u);
     for (u32 s = 0; s < n_seg; ++s) {
       int rc = launch_w(b, threads, blocks, (u32)ord.size(), s, s * seg, (s + 1) * seg);
@@ -873,6 +881,20 @@ int ppvm_batch_results(ppvm_batch_handle b, uint32_t* status, uint64_t* n_final,
     if (overlap_last) overlap_last[i] = I.ovl_last;
     if (term_updates) term_updates[i] = I.term_updates;
     if (peak) peak[i] = I.peak;
+  }
+  return PPVM_OK;
+}
+
+int ppvm_batch_latency(ppvm_batch_handle b, double* seconds) {
+  if (!b || !seconds) return bfail(PPVM_EINVAL, "null argument");
+  u64 t0 = 0;
+  BCHK(hipMemcpy(&t0, b->d_t0, sizeof(u64), hipMemcpyDeviceToHost));
+  int dev = 0, khz = 100000;
+  BCHK(hipGetDevice(&dev));
+  (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+  for (u32 i = 0; i < b->n; ++i) {
+    const BInst& I = b->h_inst[i];
+    seconds[i] = I.t_done > t0 ? (double)(I.t_done - t0) / ((double)khz * 1e3) : 0.0;
   }
   return PPVM_OK;
 }
