@@ -259,6 +259,33 @@ int ppvm_batch_marks(ppvm_batch_handle b, size_t i, uint64_t* lens, double* over
 int ppvm_batch_export(ppvm_batch_handle b, size_t i, uint64_t* x, uint64_t* z, double* c, size_t cap,
                       size_t* n);
 
+/* ---- learned surrogate ---------------------------------------------------------
+ * No counterpart in the reference.  An ensemble of MLPs (in -> H -> H -> H -> 1,
+ * GELU) with BF16 weights/activations and FP32 accumulation on MFMA, trained
+ * with Adam (FP32 master weights) on expectation values produced by the
+ * propagator.  Parameter vector layout per member, for l = 0..3:
+ * W_l [N_l][K_l] then b_l [N_l], with K = (32, H, H, H), N = (H, H, H, 32);
+ * inputs beyond in_dim and output rows beyond 0 are padding (kept zero). */
+typedef struct ppvm_mlp* ppvm_mlp_handle;
+const char* ppvm_mlp_last_error(void);
+int ppvm_mlp_create(uint32_t in_dim, uint32_t hidden, uint32_t members, uint32_t max_batch, uint64_t seed,
+                    ppvm_mlp_handle* out);
+int ppvm_mlp_destroy(ppvm_mlp_handle h);
+uint64_t ppvm_mlp_param_count(ppvm_mlp_handle h);
+int ppvm_mlp_get_params(ppvm_mlp_handle h, uint32_t member, float* out, size_t cap);
+int ppvm_mlp_set_params(ppvm_mlp_handle h, uint32_t member, const float* in, size_t n);
+/* Full-batch training (MSE), `steps` Adam steps with cosine decay from lr;
+ * final_loss receives the last step's loss per member. */
+int ppvm_mlp_train(ppvm_mlp_handle h, const float* X, const float* Y, uint32_t n, uint32_t steps, float lr,
+                   uint32_t use_graph, float* final_loss);
+/* Ensemble mean and standard deviation per row (per_member: members x n, may be NULL). */
+int ppvm_mlp_predict(ppvm_mlp_handle h, const float* X, uint32_t n, float* mean, float* sd, float* per_member);
+/* One forward + backward pass: gradients of the MSE loss for one member. */
+int ppvm_mlp_grads(ppvm_mlp_handle h, const float* X, const float* Y, uint32_t n, uint32_t member, float* grads,
+                   size_t cap, float* loss);
+/* C[M][N] = bf16(A)[M][K] * bf16(B)[N][K]^T through the MFMA GEMM kernel (tests). */
+int ppvm_mlp_gemm_check(const float* A, const float* B, int M, int N, int K, float* C);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,82 @@
+"""CPU checks of host-side components: workload generators, sweep features,
+and the MLP reference (pinned against finite differences)."""
+
+import numpy as np
+
+from ppvm_amd import ops as O
+from ppvm_amd import workloads as WL
+
+
+def test_workloads_are_deterministic_and_shaped():
+    a, b = WL.lattice_tfim(), WL.lattice_tfim()
+    assert np.array_equal(a.program, b.program)
+    assert a.n_gates == 20 * (90 + 90 + 100)
+    assert len(a.layer_marks) == 60
+    # backward order: the first ops are rx on qubits 99..0, each followed by a truncate
+    assert [int(c) for c in a.program["code"][:4]] == [O.RX, O.TRUNCATE, O.RX, O.TRUNCATE]
+    assert [int(q) for q in a.program["q0"][:4:2]] == [99, 98]
+    assert a.observable == [("I" * 55 + "Z" + "I" * 44, 1.0)]
+    c = WL.chain_tfim()
+    assert c.n_gates == 10 * (19 + 20) and c.observable[0][0][10] == "Z"
+
+
+def test_heavy_hex_graph_and_random_circuit():
+    n, edges = WL.heavy_hex_graph()
+    assert n == 127 and len(edges) == 144
+    deg = np.zeros(n, int)
+    for a, b in edges:
+        deg[a] += 1
+        deg[b] += 1
+    assert deg.max() <= 3 and deg.min() >= 1
+    w = WL.random_circuit()
+    w2 = WL.random_circuit()
+    assert np.array_equal(w.program, w2.program)
+    assert len(w.params["obs"]) == 3 and len(set(w.params["obs"])) == 3
+    codes = set(int(c) for c in w.program["code"])
+    assert codes <= {O.CNOT, O.H, O.S, O.RZ, O.TRUNCATE}
+
+
+def test_batched_chain_parameters():
+    ws = WL.batched_chains()
+    assert len(ws) == 256
+    Ls = [w.n_qubits for w in ws]
+    assert min(Ls) >= 32 and max(Ls) <= 128
+    steps = [w.params["steps"] for w in ws]
+    assert min(steps) >= 5 and max(steps) <= 30
+    hs = [w.params["h"] for w in ws]
+    assert 0.5 <= min(hs) and max(hs) < 1.5
+
+
+def test_sweep_features():
+    from ppvm_amd.surrogate import features
+    f = features([0.5, 4.0], [0.0, 2.0])
+    assert f.shape == (2, 11)
+    assert np.allclose(f[:, 0], [-1.0, 1.0])
+    assert np.allclose(f[:, 2], [-1.0, 1.0])
+
+
+def test_mlp_reference_matches_finite_differences():
+    from oracle.mlp_ref import forward_backward, shapes
+    rng = np.random.default_rng(0)
+    hidden = 8
+    K, N = shapes(hidden)
+    P = sum(n * k + n for k, n in zip(K, N))
+    p = rng.normal(scale=0.3, size=P)
+    X = rng.uniform(-1, 1, size=(20, 5))
+    Y = rng.normal(size=20)
+    loss, g = forward_backward(p, X, Y, hidden, rounding=False)
+    idx = rng.choice(P, size=40, replace=False)
+    for i in idx:
+        e = np.zeros(P)
+        e[i] = 1e-6
+        lp, _ = forward_backward(p + e, X, Y, hidden, rounding=False)
+        lm, _ = forward_backward(p - e, X, Y, hidden, rounding=False)
+        fd = (lp - lm) / 2e-6
+        assert abs(fd - g[i]) <= 1e-6 + 1e-5 * abs(fd), (i, fd, g[i])
+
+
+def test_bf16_rounding_is_nearest_even():
+    from oracle.mlp_ref import bf16
+    x = np.array([1.0, 1.0 + 2**-8, 1.0 + 3 * 2**-8, -2.5, 1e-40], dtype=np.float32)
+    r = bf16(x)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0 + 4 * 2**-8) and r[3] == -2.5
