@@ -366,6 +366,32 @@ int launch_overlap(ppvm_sum* s) {
   return PPVM_OK;
 }
 
+// Σ_{k in b} c_b(k) c_a(k): a's live keys into a's table, then one probe per
+// live term of b, all on a's stream (b is synchronised by the caller).
+template <int W>
+int launch_overlap2(ppvm_sum* a, ppvm_sum* b) {
+  int rc;
+  const u32 ia = (u32)(a->op_seq & 1), ib = (u32)(b->op_seq & 1);
+  const u32 tg = next_tab_gen(a);
+  const u64 tmask = a->tcap - 1;
+  if ((rc = prof_begin(a, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_ovl_build<W>, dim3(grid_for(a->n_bound, kBlock * 4, kFixedGrid)), dim3(kBlock), 0,
+                     a->stream, a->planes[a->cur], a->coef[a->cur], a->cap, a->st, ia, a->table, tmask, tg);
+  if ((rc = prof_end(a))) return rc;
+  if ((rc = prof_begin(a, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_ovl_probe<W>, dim3(kOverlapGrid), dim3(kBlock), 0, a->stream, a->planes[a->cur],
+                     a->coef[a->cur], a->cap, a->table, tmask, tg, b->planes[b->cur], b->coef[b->cur], b->cap,
+                     b->st, ib, a->partials);
+  if ((rc = prof_end(a))) return rc;
+  FinishArgs fa{};
+  fa.mode = F_OVERLAP;
+  fa.Ga = kOverlapGrid;
+  fa.dpart = a->partials;
+  if ((rc = launch_finish(a, fa))) return rc;
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
 template <int W>
 int launch_compact(ppvm_sum* s) {
   int rc;
@@ -399,6 +425,7 @@ int issue_rot_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_rot, 
 int issue_map_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_map, it); }
 int overlap_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_overlap); }
 int compact_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_compact); }
+int overlap2_w(ppvm_sum* s, ppvm_sum* b) { PPVM_DISPATCH_W(s, launch_overlap2, b); }
 
 int issue(ppvm_sum* s, Issued& it, bool log_it) {
   int rc = it.kind == IK_ROT ? issue_rot_w(s, it) : issue_map_w(s, it);
@@ -1083,7 +1110,18 @@ int ppvm_overlap_with_zero(ppvm_handle h, double* out) {
 int ppvm_overlap(ppvm_handle a, ppvm_handle b, double* out) {
   if (!a || !b || !out) return fail(PPVM_EINVAL, "null argument");
   if (a->n != b->n) return fail(PPVM_EINVAL, "overlap of sums over %u and %u qubits", a->n, b->n);
-  return fail(PPVM_EINVAL, "ppvm_overlap between two sums is not available in this build");
+  if (a->device != b->device)
+    return fail(PPVM_EINVAL, "overlap of sums on devices %d and %d", a->device, b->device);
+  int rc;
+  if ((rc = ppvm_synchronize(a))) return rc;
+  if (b != a && (rc = ppvm_synchronize(b))) return rc;
+  if ((rc = overlap2_w(a, b))) {
+    a->broken = true;
+    return rc;
+  }
+  if ((rc = read_state(a))) return rc;
+  *out = a->st_host->overlap;
+  return PPVM_OK;
 }
 
 int ppvm_export_terms(ppvm_handle h, uint64_t* x, uint64_t* z, double* c, size_t cap, size_t* n) {

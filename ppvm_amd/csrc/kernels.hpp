@@ -565,6 +565,86 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ plan
   if (threadIdx.x == 0) partials[blockIdx.x] = lds_d[0];
 }
 
+// ---- overlap of two sums: sum over b's live terms of c_b(k) * c_a(k) ------------
+// Build pass: every live entry of a is inserted (key hash -> arena index) into
+// a's pair table under a fresh generation.  Live keys of an arena are unique,
+// and the table holds at least twice the arena capacity, so each probe ends.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_ovl_build(const u64* __restrict__ planes,
+                                                      const double* __restrict__ coef, u64 cap,
+                                                      const DevState* __restrict__ st, u32 in_slot,
+                                                      u64* table, u64 tmask, u32 tgen) {
+  const u64 n = st->slot[in_slot].n_entries;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    if (is_dead(coef[i])) continue;
+    u64 k[2 * W];
+    load_key<W>(planes, cap, i, k);
+    const u64 h = hash_key<W>(k);
+    const u64 mine = ((u64)tgen << 40) | ((h >> 56) << 32) | i;
+    u64 pos = h & tmask;
+    u64 s = table[pos];
+    while (true) {
+      if ((u32)(s >> 40) != tgen) {
+        const u64 old = cas_u64(&table[pos], s, mine);
+        if (old == s) break;
+        s = old;
+        continue;
+      }
+      pos = (pos + 1) & tmask;
+      s = table[pos];
+    }
+  }
+}
+
+// Probe pass over b: per-block partial sums of c_b * c_a for keys present in a.
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_ovl_probe(const u64* __restrict__ pa,
+                                                      const double* __restrict__ ca, u64 cap_a,
+                                                      const u64* __restrict__ table, u64 tmask, u32 tgen,
+                                                      const u64* __restrict__ pb,
+                                                      const double* __restrict__ cb, u64 cap_b,
+                                                      const DevState* __restrict__ st_b, u32 slot_b,
+                                                      double* __restrict__ partials) {
+  __shared__ double lds_d[kBlock];
+  const u64 n = st_b->slot[slot_b].n_entries;
+  double acc = 0.0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double c = cb[i];
+    if (is_dead(c)) continue;
+    u64 k[2 * W];
+    load_key<W>(pb, cap_b, i, k);
+    const u64 h = hash_key<W>(k);
+    const u64 fp = h >> 56;
+    u64 pos = h & tmask;
+    while (true) {
+      const u64 s = table[pos];
+      if ((u32)(s >> 40) != tgen) break;  // not in a
+      if (((s >> 32) & 0xffull) == fp) {
+        const u64 j = s & 0xffffffffull;
+        u64 kj[2 * W];
+        load_key<W>(pa, cap_a, j, kj);
+        bool same = true;
+#pragma unroll
+        for (int p = 0; p < 2 * W; ++p) same &= (kj[p] == k[p]);
+        if (same) {
+          acc = __dadd_rn(acc, __dmul_rn(c, ca[j]));
+          break;
+        }
+      }
+      pos = (pos + 1) & tmask;
+    }
+  }
+  lds_d[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) lds_d[threadIdx.x] = __dadd_rn(lds_d[threadIdx.x], lds_d[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = lds_d[0];
+}
+
 // ---- dead-entry compaction into the other arena --------------------------------
 __global__ __launch_bounds__(kBlock) void k_compact_count(const double* __restrict__ coef,
                                                           const DevState* __restrict__ st, u32 in_slot,

@@ -233,15 +233,18 @@ __global__ void k_ensemble(const float* out, long long ldo, long long zo, int n,
                            float* sd) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float s = 0.f, s2 = 0.f;
-  for (int z = 0; z < members; ++z) {
-    const float v = out[z * zo + (long long)i * ldo];
-    s += v;
-    s2 += v * v;
-  }
+  // two passes (mean, then squared deviations): the spread is small next to
+  // the mean, and E[v^2] - mu^2 would cancel most of its digits
+  float s = 0.f;
+  for (int z = 0; z < members; ++z) s += out[z * zo + (long long)i * ldo];
   const float mu = s / members;
+  float s2 = 0.f;
+  for (int z = 0; z < members; ++z) {
+    const float dv = out[z * zo + (long long)i * ldo] - mu;
+    s2 += dv * dv;
+  }
   mean[i] = mu;
-  sd[i] = sqrtf(fmaxf(0.f, s2 / members - mu * mu));
+  sd[i] = sqrtf(s2 / members);
 }
 
 u64 sm64(u64& s) {

@@ -156,3 +156,48 @@ def test_determinism():
     ea, eb = a._export(), b._export()
     assert all(np.array_equal(u, v) for u, v in zip(ea, eb))
     assert a.overlap_with_zero() == b.overlap_with_zero()
+
+
+def test_overlap_known_answers():
+    """crates/ppvm-pauli-sum/src/sum/trace.rs:52-80 on the GPU."""
+    from ppvm_amd import PauliSum
+
+    def mk(terms):
+        return PauliSum([t for t, _ in terms], 4, coefficients=[c for _, c in terms])
+
+    a = mk([("IIII", 1.0), ("XIII", 2.0)])
+    empty = mk([("XIII", 1e-20)])  # the constructor needs one term; truncation removes it
+    empty.truncate()
+    assert len(empty) == 0
+    assert a.overlap(empty) == 0.0 and empty.overlap(a) == 0.0
+    s = mk([("XIII", 3.0)])
+    assert s.overlap(s) == 9.0
+    assert mk([("XIII", 1.0)]).overlap(mk([("YIII", 1.0)])) == 0.0
+    assert a.overlap(mk([("IIII", 3.0), ("XIII", 4.0)])) == 11.0
+
+
+@pytest.mark.parametrize("n", [6, 70, 130, 300])
+def test_overlap_between_sums(n):
+    """Two sums propagated by the same truncated circuit from different
+    (overlapping) starting sets: overlap(a, b), overlap(b, a), overlap(a, a)
+    against the oracle (relative 1e-12: the GPU sums in another association)."""
+    rng = np.random.default_rng(300 + n)
+    strings = random_terms(rng, n, 120)
+    prog = random_program(rng, n, 60, trunc_every=True)
+    gs, rs = [], []
+    for side in range(2):
+        sel = strings[side * 40: side * 40 + 80]
+        g, r = _pair(n, sel, rng.normal(size=len(sel)).tolist(), tau=1e-3)
+        g.apply(prog)
+        r.apply(prog)
+        assert len(g) == len(r)
+        gs.append(g)
+        rs.append(r)
+    for i, j in ((0, 1), (1, 0), (0, 0)):
+        og, oo = gs[i].overlap(gs[j]), rs[i].overlap(rs[j])
+        assert abs(og - oo) <= 1e-12 * max(1.0, abs(oo)), (i, j, og, oo)
+    # after further gates on one side only
+    gs[0].h(0)
+    rs[0].apply(O.Program().add(O.H, 0).array())
+    og, oo = gs[0].overlap(gs[1]), rs[0].overlap(rs[1])
+    assert abs(og - oo) <= 1e-12 * max(1.0, abs(oo))

@@ -343,3 +343,39 @@ def test_sharded_variant_matches_single_thread():
     b.apply_sharded(w.program, 4)
     assert dict(a.terms()) == dict(b.terms())
     assert a.term_updates() == b.term_updates()
+
+
+# ---- overlap between two sums ------------------------------------------------------------------
+def test_overlap_known_answers():
+    """crates/ppvm-pauli-sum/src/sum/trace.rs:52-80."""
+    a = _sum(4, [("IIII", 1.0), ("XIII", 2.0)])
+    empty = _sum(4, [])
+    assert a.overlap(empty) == 0.0 and empty.overlap(a) == 0.0
+    s = _sum(4, [("XIII", 3.0)])
+    assert s.overlap(s) == 9.0
+    assert _sum(4, [("XIII", 1.0)]).overlap(_sum(4, [("YIII", 1.0)])) == 0.0
+    b = _sum(4, [("IIII", 3.0), ("XIII", 4.0)])
+    assert a.overlap(b) == 1.0 * 3.0 + 2.0 * 4.0
+
+
+def test_overlap_is_hilbert_schmidt_product():
+    """overlap(A, B) = Tr(A B) / 2^n for real Pauli sums, after random circuits."""
+    rng = np.random.default_rng(12)
+    n = 4
+    for trial in range(4):
+        sums = []
+        for side in range(2):
+            strs = list({"".join("IXYZ"[v] for v in rng.integers(0, 4, n)) for _ in range(12)})
+            s = _sum(n, list(zip(strs, rng.normal(size=len(strs)).tolist())))
+            p = O.Program()
+            for _ in range(10):
+                a, b = rng.choice(n, 2, replace=False)
+                code = [O.H, O.CNOT, O.RX, O.RZZ, O.RY][rng.integers(5)]
+                p.add(code, int(a), int(b), theta=float(rng.uniform(-2, 2)))
+            s.apply(p.array())
+            sums.append(s)
+        A = sum(c * pmat(t) for t, c in sums[0].terms())
+        B = sum(c * pmat(t) for t, c in sums[1].terms())
+        hs = np.trace(A @ B).real / 2**n
+        assert abs(sums[0].overlap(sums[1]) - hs) < 1e-12
+        assert abs(sums[1].overlap(sums[0]) - hs) < 1e-12
