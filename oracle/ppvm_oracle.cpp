@@ -26,7 +26,10 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -444,6 +447,64 @@ struct Shard {
   Sum m;
 };
 
+// Persistent workers: run(fn) executes fn(t) for t = 0..S-1 (t = 0 on the
+// calling thread) and returns when all are done.
+class Pool {
+ public:
+  explicit Pool(int S) : S_(S) {
+    for (int t = 1; t < S_; ++t) th_.emplace_back([this, t] { loop(t); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      stop_ = true;
+      ++gen_;
+    }
+    cv_.notify_all();
+    for (auto& x : th_) x.join();
+  }
+  void run(const std::function<void(int)>& fn) {
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      job_ = &fn;
+      left_ = S_ - 1;
+      ++gen_;
+    }
+    cv_.notify_all();
+    fn(0);
+    std::unique_lock<std::mutex> g(mu_);
+    done_.wait(g, [this] { return left_ == 0; });
+  }
+
+ private:
+  void loop(int t) {
+    uint64_t seen = 0;
+    while (true) {
+      const std::function<void(int)>* job;
+      {
+        std::unique_lock<std::mutex> g(mu_);
+        cv_.wait(g, [&] { return gen_ != seen; });
+        seen = gen_;
+        if (stop_) return;
+        job = job_;
+      }
+      (*job)(t);
+      {
+        std::lock_guard<std::mutex> g(mu_);
+        if (--left_ == 0) done_.notify_one();
+      }
+    }
+  }
+  int S_;
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)>* job_ = nullptr;
+  int left_ = 0;
+  uint64_t gen_ = 0;
+  bool stop_ = false;
+};
+
 int apply_sharded(Sum& src, const ppvm_op* ops, size_t nops, int threads) {
   if (threads < 1) threads = 1;
   const int S = threads;
@@ -460,11 +521,10 @@ int apply_sharded(Sum& src, const ppvm_op* ops, size_t nops, int threads) {
   uint64_t tu = 0;
   std::vector<std::vector<std::vector<uint64_t>>> outk(S, std::vector<std::vector<uint64_t>>(S));
   std::vector<std::vector<std::vector<double>>> outc(S, std::vector<std::vector<double>>(S));
+  Pool pool(S);
   auto par = [&](auto fn) {
-    std::vector<std::thread> th;
-    for (int t = 1; t < S; ++t) th.emplace_back(fn, t);
-    fn(0);
-    for (auto& x : th) x.join();
+    const std::function<void(int)> f = fn;
+    pool.run(f);
   };
   for (size_t oi = 0; oi < nops; ++oi) {
     const ppvm_op& op = ops[oi];

@@ -558,9 +558,13 @@ int ppvm_mlp_train(ppvm_mlp_handle h, const float* X, const float* Y, uint32_t n
   if ((rc = stage_inputs(h, X, Y, n))) return rc;
   h->total_steps = (int)std::max<u32>(1, steps);
   int zero = 0;
-  MCHK(hipMemcpyAsync(h->step, &zero, 4, hipMemcpyHostToDevice, h->stream));
-  MCHK(hipStreamSynchronize(h->stream));
   const long long P = h->per_member * h->members;
+  // each call is a fresh Adam run: step counter and both moments start at zero
+  MCHK(hipMemcpyAsync(h->step, &zero, 4, hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(k_zero_f32, dim3(512), dim3(256), 0, h->stream, h->m, P);
+  hipLaunchKernelGGL(k_zero_f32, dim3(512), dim3(256), 0, h->stream, h->v, P);
+  MCHK(hipGetLastError());
+  MCHK(hipStreamSynchronize(h->stream));
   AdamArgs a{h->p, h->g, h->m, h->v, P, lr, 0.9f, 0.999f, 1e-8f, h->step, h->total_steps};
   auto one = [&](hipStream_t st) -> int {
     int r;

@@ -12,6 +12,7 @@ propagation per distinct h, prefix reuse over t).
 from __future__ import annotations
 
 import ctypes
+import os
 import time
 from dataclasses import dataclass
 
@@ -60,14 +61,16 @@ def _f32(a):
 
 
 def features(h, t, dt: float = DT) -> np.ndarray:
-    """Network inputs for sweep points (h, t): scaled h, t and step count, and
-    sin/cos of h*t_s at a few harmonics and of pi*t_s, where t_s = steps*dt is
-    the time the Trotter circuit actually reaches."""
+    """Network inputs for sweep points (h, t): scaled h, time and step count,
+    and sin/cos of h*t_s at harmonics 1, 2, 4 and of pi*t_s.  Time enters as
+    t_s = steps*dt, the time the Trotter circuit actually reaches
+    (steps = round(t/dt)): grid points with equal step counts are the same
+    physical point and get the same inputs."""
     h = np.asarray(h, dtype=np.float64)
     t = np.asarray(t, dtype=np.float64)
     steps = np.rint(t / dt)
     ts = steps * dt
-    cols = [(h - 2.25) / 1.75, t - 1.0, steps / 20.0 - 1.0]
+    cols = [(h - 2.25) / 1.75, ts - 1.0, steps / 20.0 - 1.0]
     for k in (1.0, 2.0, 4.0):
         cols += [np.sin(k * h * ts), np.cos(k * h * ts)]
     cols += [np.sin(np.pi * ts), np.cos(np.pi * ts)]
@@ -143,6 +146,18 @@ def gemm_check(A, B) -> np.ndarray:
     return C
 
 
+GRID_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "lattice_sweep_grid.npz")
+
+
+def load_grid(path: str = GRID_FILE) -> dict:
+    """Propagator grid of the lattice TFIM sweep written by ``python -m
+    ppvm_amd.sweep --out FILE``: h (64), t (64), dt, grid[h, m] = overlap after
+    m backward Trotter steps (m = 0..40), seconds[h] = propagation wall time per
+    h value on one MI355X, term_updates[h]."""
+    with np.load(path, allow_pickle=False) as d:
+        return {k: d[k] for k in d.files}
+
+
 @dataclass
 class SweepReport:
     n_query: int
@@ -158,10 +173,12 @@ class SweepReport:
     plain_s: float             # propagator sweep over the same query points
     speedup: float
     train_loss: list
+    abs_err_net: np.ndarray    # network alone (before fallback), per query point
+    fallback_live: bool        # fallbacks propagated live (else charged at recorded times)
 
 
 def run_config5(grid: np.ndarray, h_values: np.ndarray, t_values: np.ndarray, plain_seconds: np.ndarray,
-                n_train: int = 2048, seed: int = 31, members: int = 4, steps: int = 4000, lr: float = 2e-3,
+                n_train: int = 2048, seed: int = 31, members: int = 4, steps: int = 40000, lr: float = 1e-3,
                 fallback_std: float = 1e-2, propagate=None, log=None) -> SweepReport:
     """Surrogate sweep of the (h, t) grid: train on ``n_train`` seed-sampled
     points, infer the rest, fall back to the propagator where the ensemble std
@@ -190,6 +207,7 @@ def run_config5(grid: np.ndarray, h_values: np.ndarray, t_values: np.ndarray, pl
     infer_s = time.perf_counter() - t0
     fb = sd > fallback_std
     pred = mean.astype(np.float64)
+    err_net = np.abs(pred - Yall[query_idx])
     hi = (query_idx // len(t_values))
     fb_h = np.unique(hi[fb])
     t0 = time.perf_counter()
@@ -199,6 +217,9 @@ def run_config5(grid: np.ndarray, h_values: np.ndarray, t_values: np.ndarray, pl
         sel = (hi == i) & fb
         pred[sel] = ser[steps_grid.ravel()[query_idx[sel]]]
     fallback_s = time.perf_counter() - t0
+    if propagate is None:
+        # no live propagator: charge the recorded full-range time of each fallback h
+        fallback_s = float(sum(plain_seconds[i] for i in fb_h))
     err = np.abs(pred - Yall[query_idx])
     q_h = np.unique(hi)
     plain_s = float(sum(plain_seconds[i] for i in q_h))
@@ -206,4 +227,5 @@ def run_config5(grid: np.ndarray, h_values: np.ndarray, t_values: np.ndarray, pl
     net.close()
     return SweepReport(len(query_idx), int(fb.sum()), len(fb_h), err, float(np.mean(err <= 1e-3)),
                        float(err.max()), train_s, infer_s, fallback_s, surrogate_s, plain_s,
-                       plain_s / surrogate_s if surrogate_s > 0 else float("inf"), loss.tolist())
+                       plain_s / surrogate_s if surrogate_s > 0 else float("inf"), loss.tolist(),
+                       err_net, propagate is not None)

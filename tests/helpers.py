@@ -2,6 +2,8 @@
 
 from __future__ import annotations
 
+import functools
+
 import numpy as np
 
 from oracle.pyoracle import OracleSum
@@ -89,3 +91,49 @@ def random_program(rng, n, ngates, trunc_every=True, kinds=None):
         if trunc_every:
             p.truncate()
     return p.array()
+
+
+def fingerprint(export) -> dict:
+    """Order-independent fingerprint of a term set: count, SHA-256 of the keys
+    sorted lexicographically (x words, then z words) and of the coefficient
+    bits in that order."""
+    import hashlib
+    x, z, c = export
+    x = np.asarray(x, dtype=np.uint64)
+    z = np.asarray(z, dtype=np.uint64)
+    if x.ndim == 1:  # flat term-major words
+        x = x.reshape(len(c), -1) if len(c) else x.reshape(0, 1)
+        z = z.reshape(len(c), -1) if len(c) else z.reshape(0, 1)
+    k = np.concatenate([x, z], axis=1)
+    order = np.lexsort(k.T[::-1]) if len(c) else np.zeros(0, dtype=np.int64)
+    ks = np.ascontiguousarray(k[order])
+    cs = np.ascontiguousarray(np.asarray(c, dtype=np.float64)[order])
+    return {"terms": int(len(c)),
+            "keys": hashlib.sha256(ks.tobytes()).hexdigest()[:32],
+            "coefs": hashlib.sha256(cs.view(np.uint64).tobytes()).hexdigest()[:32]}
+
+
+@functools.lru_cache(maxsize=1)
+def fingerprint_workloads():
+    """The workloads whose per-layer fingerprints are committed as fixtures:
+    name -> Workload."""
+    from ppvm_amd import workloads as WL
+    ws = {"chain_tfim_n20": WL.chain_tfim(),
+          "lattice_tfim_10x10_prefix6": WL.truncate_steps(WL.lattice_tfim(), 6),
+          "random_circuit_127q": WL.random_circuit()}
+    chains = WL.batched_chains()
+    for i in (0, 1, 2, 3):
+        ws[f"batched_chain_{i}"] = chains[i]
+    return ws
+
+
+def layer_fingerprints(sum_, w, export_fn) -> list:
+    """Apply w layer by layer on sum_ and fingerprint after every layer."""
+    out, prev = [], 0
+    for m in w.layer_marks:
+        sum_.apply(w.program[prev:m + 1])
+        prev = m + 1
+        fp = fingerprint(export_fn())
+        fp["overlap"] = float(sum_.overlap_with_zero())
+        out.append(fp)
+    return out

@@ -53,6 +53,8 @@ def test_sweep_features():
     assert f.shape == (2, 11)
     assert np.allclose(f[:, 0], [-1.0, 1.0])
     assert np.allclose(f[:, 2], [-1.0, 1.0])
+    g = features([1.3, 1.3], [0.99, 1.01])  # both round to 20 steps: the same physical point
+    assert np.array_equal(g[0], g[1])
 
 
 def test_mlp_reference_matches_finite_differences():
@@ -80,3 +82,25 @@ def test_bf16_rounding_is_nearest_even():
     x = np.array([1.0, 1.0 + 2**-8, 1.0 + 3 * 2**-8, -2.5, 1e-40], dtype=np.float32)
     r = bf16(x)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0 + 4 * 2**-8) and r[3] == -2.5
+
+
+def test_shipped_sweep_grid_matches_oracle_prefix():
+    """The propagator grid shipped for the surrogate (written on the GPU by
+    `python -m ppvm_amd.sweep`) agrees with the oracle over the first Trotter
+    steps at the smallest and largest h (1e-10 absolute)."""
+    from ppvm_amd import workloads as WL
+    from ppvm_amd.surrogate import load_grid
+    from tests.helpers import oracle_for
+    d = load_grid()
+    assert d["grid"].shape == (64, 41)
+    assert np.all(d["grid"][:, 0] == 1.0)
+    for i in (0, 63):
+        w = WL.lattice_tfim(h=float(d["h"][i]), steps=9)
+        o = oracle_for(w)
+        vals, prev = [o.overlap_with_zero()], 0
+        for li, m in enumerate(w.layer_marks):
+            o.apply(w.program[prev:m + 1])
+            prev = m + 1
+            if li % 3 == 2:
+                vals.append(o.overlap_with_zero())
+        assert np.max(np.abs(np.array(vals) - d["grid"][i, :10])) <= 1e-10
