@@ -50,12 +50,15 @@ KERNELS = ["k_rot_count", "k_rot_gather", "k_rot_pair", "k_emit_count", "k_emit_
            "k_finish", "other"]
 
 
-def phase_bytes(cnt: dict, W: int) -> dict:
+def phase_bytes(cnt: dict, W: int, nq: int) -> dict:
     """Algorithmic HBM bytes per kernel, summed over the run (S = 16W + 8 B per
     term, R = 16W + 16 B per worklist record, N = scanned entries, N_a =
-    anticommuting terms, N_p = merged pairs, N_c = appended children):
+    anticommuting terms, N_p = merged pairs, N_c = appended children, N_t =
+    entries transposed into the bit-sliced planes):
 
-    k_rot_count  : 8 B per test word of every entry + N/8 B of ballot masks.
+    k_rot_count  : 1 bit per bit-sliced plane the generator needs for every
+                   entry + N/8 B of ballot masks + N_t (16W + 2 nq / 8) for the
+                   entries tested from the key planes and transposed.
     k_rot_gather : N/8 B masks + N_a (S + R): key and coefficient read, record written.
     k_rot_pair   : N_a R records read + 2 N_p 8 B coefficients written.
     k_emit_count : N_a R records read + (N_a - 2 N_p) 8 B coefficients written + N_a/8 masks.
@@ -68,7 +71,7 @@ def phase_bytes(cnt: dict, W: int) -> dict:
     npair = cnt["pairs_total"]
     nch = cnt["children_total"]
     return {
-        "k_rot_count": 8 * cnt["scan_words"] + N / 8,
+        "k_rot_count": cnt["scan_planes"] / 8 + N / 8 + cnt["transposed"] * (16 * W + 2 * nq / 8),
         "k_rot_gather": N / 8 + na * (S + R),
         "k_rot_pair": na * R + 2 * npair * 8,
         "k_emit_count": na * R + (na - 2 * npair) * 8 + na / 8,
@@ -129,7 +132,7 @@ def run(args) -> dict:
     ms, launches = g._h.profile_read()
     cnt = g._h.counters()
     del g
-    pb = phase_bytes(cnt, W)
+    pb = phase_bytes(cnt, W, w.n_qubits)
     kern = {}
     for i, nm in enumerate(KERNELS):
         if launches[i]:
@@ -373,6 +376,7 @@ def run_surrogate(args) -> dict:
                     "surrogate_query": rep.surrogate_s, "plain_query": rep.plain_s,
                     "plain_full_grid": float(secs.sum())},
         "train_loss": rep.train_loss,
+        "std_bound_tradeoff": rep.tradeoff,
         "grid_check": check,
     }
 

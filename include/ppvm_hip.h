@@ -194,9 +194,10 @@ int ppvm_get_stats(ppvm_handle h, ppvm_stats* out);
 /* Device counters for the measurement harness, in this order: term_updates,
  * gates, rotations, scanned entries, anticommuting (worklist) terms, merged
  * pairs, appended children, truncated terms, map-program entries, map
- * launches, scan plane-words, capacity, syncs, regrows, compactions,
- * peak live terms. */
-int ppvm_get_counters(ppvm_handle h, uint64_t out[16]);
+ * launches, bit-sliced planes read by scans (entries x planes), capacity,
+ * syncs, regrows, compactions, peak live terms, entries transposed into the
+ * bit-sliced planes, reserved (0). */
+int ppvm_get_counters(ppvm_handle h, uint64_t out[18]);
 
 /* Per-phase kernel timing (HIP events on the handle's stream), for the
  * measurement harness.  Phases: 0 rotation scan, 1 rotation pair/merge,

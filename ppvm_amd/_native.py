@@ -217,11 +217,11 @@ class Handle:
         return {name: int(getattr(s, name)) for name, _ in PpvmStats._fields_}
 
     COUNTERS = ("term_updates", "gates", "rotations", "scanned", "wl_total", "pairs_total",
-                "children_total", "kills_total", "map_entries", "map_launches", "scan_words",
-                "capacity", "syncs", "regrows", "compactions", "peak_live")
+                "children_total", "kills_total", "map_entries", "map_launches", "scan_planes",
+                "capacity", "syncs", "regrows", "compactions", "peak_live", "transposed", "reserved")
 
     def counters(self) -> dict:
-        out = np.zeros(16, dtype=np.uint64)
+        out = np.zeros(18, dtype=np.uint64)
         check(lib().ppvm_get_counters(self.h, u64p(out)))
         return {k: int(v) for k, v in zip(self.COUNTERS, out)}
 

@@ -63,9 +63,11 @@ struct alignas(64) DevState {
   u64 kills_total;    // terms truncated
   u64 map_entries;    // entries visited by map programs
   u64 map_launches;
-  u64 scan_words;     // plane words read by rotation scans (entries x test words)
+  u64 scan_planes;    // bit-sliced planes read by rotation scans (entries x planes)
   double overlap;     // last overlap_with_zero result
-  u64 pad[7];
+  u64 tvw;            // words of the bit-sliced planes that are current (entries < 64 tvw)
+  u64 transposed;     // entries written into the bit-sliced planes
+  u64 pad[5];
 };
 
 // Rotation gate exp(-i theta/2 G) with G acting on qubits a (and b).
@@ -74,7 +76,7 @@ struct RotSpec {
   u32 two;                 // 1 for a two-qubit generator
   u32 gax, gaz, gbx, gbz;  // generator Pauli bits on a and b
   u32 trunc;               // fused truncation after the gate
-  u32 test_words;          // distinct plane words the scan reads per entry
+  u32 test_planes;         // bit-sliced planes the scan reads (x/z of a, b)
   double sn, cs;           // sin(theta), cos(theta) from the host libm
 };
 

@@ -112,6 +112,8 @@ struct ppvm_sum {
   u64* table = nullptr;
   u64 tcap = 0;
   u64* masks = nullptr;     // ballot masks, cap/64 + kMaxParts words
+  u64* tp = nullptr;        // bit-sliced key planes, 2 n planes of tstride words (k_rot_count)
+  u64 tstride = 0;
   u64* parts = nullptr;     // 6 x kMaxParts per-chunk counts / partial sums
   double* partials = nullptr;
   DevState* st = nullptr;
@@ -208,9 +210,19 @@ void free_buffers(ppvm_sum* s) {
   if (s->wl) (void)hipFree(s->wl);
   if (s->table) (void)hipFree(s->table);
   if (s->masks) (void)hipFree(s->masks);
+  if (s->tp) (void)hipFree(s->tp);
   s->wl = nullptr;
   s->table = nullptr;
   s->masks = nullptr;
+  s->tp = nullptr;
+}
+
+// The bit-sliced planes no longer match the arena (new buffers, rewritten or
+// moved keys): the next scan rebuilds them.
+int invalidate_tp(ppvm_sum* s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, &s->st->tvw, 1ull, 0ull);
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
 }
 
 // (Re)allocate all capacity-dependent buffers for capacity new_cap, keeping the
@@ -250,13 +262,17 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
   if (s->wl) (void)hipFree(s->wl);
   if (s->table) (void)hipFree(s->table);
   if (s->masks) (void)hipFree(s->masks);
+  if (s->tp) (void)hipFree(s->tp);
   s->wl = nullptr;
   s->table = nullptr;
   s->masks = nullptr;
+  s->tp = nullptr;
   s->tcap = pow2_at_least(2 * new_cap);
+  s->tstride = new_cap / 64 + 8;  // multiple of 8 words; room for the last 8-word group
   if (hipMalloc(&s->wl, new_cap * s->wl_rec()) != hipSuccess ||
       hipMalloc(&s->table, s->tcap * 8) != hipSuccess ||
-      hipMalloc(&s->masks, (new_cap / 64 + kMaxParts + 64) * 8) != hipSuccess) {
+      hipMalloc(&s->masks, (new_cap / 64 + kMaxParts + 64) * 8) != hipSuccess ||
+      hipMalloc(&s->tp, 2 * (size_t)s->n * s->tstride * 8) != hipSuccess) {
     (void)hipGetLastError();
     return fail(PPVM_ENOMEM, "device allocation of work buffers for %llu terms failed",
                 (unsigned long long)new_cap);
@@ -266,6 +282,8 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
   HIPCHK(hipGetLastError());
   s->tab_gen = 0;
   s->cap = new_cap;
+  int rc;
+  if ((rc = invalidate_tp(s))) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   return PPVM_OK;
 }
@@ -284,6 +302,8 @@ u32 chunk_blocks(u64 bound, u32 target) {
   return (u32)g;
 }
 constexpr int kFixedGrid = 2048;  // grid-stride kernels with per-block partials
+constexpr u32 kArenaGrid = 2048;  // blocks of the chunked arena passes (count, gather)
+constexpr u32 kListGrid = 1024;   // blocks of the chunked worklist passes (emit)
 
 int launch_finish(ppvm_sum* s, const FinishArgs& a) {
   int rc;
@@ -296,14 +316,16 @@ template <int W>
 int launch_rot(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
   int rc;
-  const u32 G = chunk_blocks(s->n_bound, kArenaChunk);
-  const u32 Gl = chunk_blocks(s->n_bound, kListChunk);
+  // chunked kernels loop over their chunks, so the grids stay near one
+  // resident wave of blocks however large the arena is
+  const u32 G = std::min<u32>(chunk_blocks(s->n_bound, kArenaChunk), kArenaGrid);
+  const u32 Gl = std::min<u32>(chunk_blocks(s->n_bound, kListChunk), kListGrid);
   WLRec<W>* wl = reinterpret_cast<WLRec<W>*>(s->wl);
   u64* planes = s->planes[s->cur];
   double* coef = s->coef[s->cur];
   if ((rc = prof_begin(s, PH_COUNT))) return rc;
-  hipLaunchKernelGGL(k_rot_count<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, s->cap, s->st, in,
-                     it.rot, s->masks, s->part(0));
+  hipLaunchKernelGGL(k_rot_count<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, s->cap, s->tp, s->tstride,
+                     s->n, s->st, in, it.rot, s->masks, s->part(0));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_GATHER))) return rc;
   hipLaunchKernelGGL(k_rot_gather<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, coef, s->cap, s->st, in,
@@ -346,6 +368,10 @@ int launch_map(ppvm_sum* s, const Issued& it) {
   fa.n_ops = it.map.n_ops;
   if ((rc = launch_finish(s, fa))) return rc;
   HIPCHK(hipGetLastError());
+  for (u32 i = 0; i < it.map.n_ops; ++i) {
+    const u32 code = it.map.ops[i].code_flags & 0xffu;
+    if (code >= PPVM_OP_H && code <= PPVM_OP_CY) return invalidate_tp(s);  // keys changed
+  }
   return PPVM_OK;
 }
 
@@ -409,7 +435,7 @@ int launch_compact(ppvm_sum* s) {
   HIPCHK(hipGetLastError());
   s->op_seq += 1;
   s->cur ^= 1;
-  return PPVM_OK;
+  return invalidate_tp(s);
 }
 
 #define PPVM_DISPATCH_W(s, fn, ...)                  \
@@ -530,13 +556,7 @@ RotSpec make_rot(const ppvm_op& op, bool trunc) {
   }
   if (!g.two) g.b = g.a;
   g.trunc = trunc ? 1u : 0u;
-  {
-    // distinct plane words the scan reads: x words if the generator has a Z
-    // part, z words if it has an X part, for each distinct word index
-    const u32 nx = (g.gaz | g.gbz) ? 1u : 0u, nz = (g.gax | g.gbx) ? 1u : 0u;
-    const u32 nw = (g.two && (g.a >> 6) != (g.b >> 6)) ? 2u : 1u;
-    g.test_words = (nx + nz) * nw;
-  }
+  g.test_planes = g.gaz + g.gax + (g.two ? g.gbz + g.gbx : 0u);
   // Float::sin_cos of the reference (crates/ppvm-traits/src/traits/coefficient.rs:55-57):
   // the host libm, once per gate.
   g.sn = std::sin(op.theta);
@@ -805,6 +825,7 @@ int upload(ppvm_sum* s, const std::vector<u64>& keys, const std::vector<double>&
   c.n_entries = n;
   c.n_dead = 0;
   HIPCHK(hipMemcpyAsync(&s->st->slot[s->op_seq & 1], &c, sizeof(CountSlot), hipMemcpyHostToDevice, s->stream));
+  if ((rc = invalidate_tp(s))) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   s->n_bound = n;
   return PPVM_OK;
@@ -1160,16 +1181,17 @@ int ppvm_get_stats(ppvm_handle h, ppvm_stats* out) {
   return PPVM_OK;
 }
 
-int ppvm_get_counters(ppvm_handle h, uint64_t out[16]) {
+int ppvm_get_counters(ppvm_handle h, uint64_t out[18]) {
   if (!h || !out) return fail(PPVM_EINVAL, "null argument");
   int rc;
   if ((rc = ppvm_synchronize(h))) return rc;
   const DevState& d = *h->st_host;
-  const u64 v[16] = {d.term_updates, d.gates,          d.rotations,   d.scanned,
+  const u64 v[18] = {d.term_updates, d.gates,          d.rotations,   d.scanned,
                      d.wl_total,     d.pairs_total,    d.children_total, d.kills_total,
-                     d.map_entries,  d.map_launches,   d.scan_words,  h->cap,
-                     h->hs.syncs,    h->hs.regrows,    h->hs.compactions, h->hs.peak_live};
-  for (int i = 0; i < 16; ++i) out[i] = v[i];
+                     d.map_entries,  d.map_launches,   d.scan_planes, h->cap,
+                     h->hs.syncs,    h->hs.regrows,    h->hs.compactions, h->hs.peak_live,
+                     d.transposed,   0};
+  for (int i = 0; i < 18; ++i) out[i] = v[i];
   return PPVM_OK;
 }
 

@@ -75,11 +75,24 @@ __device__ __forceinline__ u64 block_sum_all(u64 v, u64* lds) {
   return t;
 }
 
+// Sum of v[0..b), returned to every thread.  Four independent loads per
+// thread are in flight per step (b reaches tens of thousands for large arenas).
+__device__ __forceinline__ u64 block_sum_prefix(const u64* __restrict__ v, u32 b, u64* lds) {
+  u64 s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  u32 j = threadIdx.x;
+  for (; j + 3u * kBlock < b; j += 4u * kBlock) {
+    s0 += v[j];
+    s1 += v[j + kBlock];
+    s2 += v[j + 2u * kBlock];
+    s3 += v[j + 3u * kBlock];
+  }
+  for (; j < b; j += kBlock) s0 += v[j];
+  return block_sum_all((s0 + s1) + (s2 + s3), lds);
+}
+
 // Sum of cnt[0..b) (the chunk's offset), returned to every thread.
 __device__ __forceinline__ u64 chunk_offset(const u64* __restrict__ cnt, u32 b, u64* lds) {
-  u64 s = 0;
-  for (u32 j = threadIdx.x; j < b; j += kBlock) s += cnt[j];
-  return block_sum_all(s, lds);
+  return block_sum_prefix(cnt, b, lds);
 }
 
 // Loads the chunk's nw mask words into lm and writes their exclusive popcount
@@ -109,37 +122,66 @@ __device__ __forceinline__ void chunk_mask_scan(const u64* __restrict__ masks, u
   __syncthreads();
 }
 
-// Anticommutation test of entry i from its test words only.
-template <int W>
-__device__ __forceinline__ u32 rot_test(const u64* __restrict__ planes, u64 cap, u64 i,
-                                        const RotSpec& g, u32 wa, u32 wb, bool needx, bool needz) {
-  u32 r = 0;
-  if (needx) {
-    const u64 xa = planes[(u64)wa * cap + i];
-    r ^= bit_of(xa, g.a) & g.gaz;
-    if (g.two) {
-      const u64 xb = (wb == wa) ? xa : planes[(u64)wb * cap + i];
-      r ^= bit_of(xb, g.b) & g.gbz;
-    }
-  }
-  if (needz) {
-    const u64 za = planes[(u64)(W + wa) * cap + i];
-    r ^= bit_of(za, g.a) & g.gax;
-    if (g.two) {
-      const u64 zb = (wb == wa) ? za : planes[(u64)(W + wb) * cap + i];
-      r ^= bit_of(zb, g.b) & g.gbx;
-    }
-  }
-  return r;
+// Position (0..63) of the k-th set bit of m (k < popcount(m)).
+__device__ __forceinline__ u32 select_bit(u64 m, u32 k) {
+  u32 pos = 0;
+  u32 c = (u32)__popcll(m & 0xffffffffull);
+  if (k >= c) { k -= c; m >>= 32; pos += 32; }
+  c = (u32)__popcll(m & 0xffffull);
+  if (k >= c) { k -= c; m >>= 16; pos += 16; }
+  c = (u32)__popcll(m & 0xffull);
+  if (k >= c) { k -= c; m >>= 8; pos += 8; }
+  c = (u32)__popcll(m & 0xfull);
+  if (k >= c) { k -= c; m >>= 4; pos += 4; }
+  c = (u32)__popcll(m & 0x3ull);
+  if (k >= c) { k -= c; m >>= 2; pos += 2; }
+  if (k >= (u32)(m & 1ull)) pos += 1;
+  return pos;
 }
 
-constexpr int kUnroll = 4;  // mask words in flight per wave
+// Chunk-local item of rank r among the set bits of the chunk's masks (after
+// chunk_mask_scan): returns the item's offset from the chunk start.  The word
+// is the last one whose exclusive prefix is <= r (words without set bits share
+// their successor's prefix, so the last such word holds the bit).
+__device__ __forceinline__ u32 chunk_rank_pos(const u64* lm, const u32* loff, u32 nw, u32 r) {
+  u32 a = 0, b = nw;
+  while (b - a > 1) {
+    const u32 mid = (a + b) >> 1;
+    if (loff[mid] <= r) a = mid; else b = mid;
+  }
+  return a * 64u + select_bit(lm[a], r - loff[a]);
+}
+
+// 64 x 64 bit transpose across a wave: lane r holds row r on entry, lane c
+// holds column c on return (bit r of the result = bit c of row r).  Six
+// butterfly stages swap the off-diagonal blocks of size 32, 16, ..., 1.
+__device__ __forceinline__ u64 transpose64(u64 x, u32 lane) {
+  const u64 lowmask[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                          0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const u32 j = 32u >> s;
+    const u64 y = __shfl_xor(x, (int)j, 64);
+    const u64 lo = lowmask[s];
+    x = (lane & j) ? ((x & ~lo) | ((y & ~lo) >> j)) : ((x & lo) | ((y & lo) << j));
+  }
+  return x;
+}
 
 // ---- k_rot_count ----------------------------------------------------------------
-// Pure test-word stream: dead entries are filtered later (their records are
-// flagged by k_rot_gather, which reads every candidate's coefficient anyway).
+// Bit-sliced planes: tp[p * tstride + w] holds, for the 64 entries of word w
+// (entry i = 64 w + bit), bit q of the x plane (p = q) or of the z plane
+// (p = nq + q).  Words w < st->tvw are current; the scan reads only the 1-4
+// planes the generator needs there (one thread per 64 entries).  Later words
+// (children appended since the last scan, or everything after a compaction,
+// upload or key-moving Clifford) are tested from the key planes, one lane per
+// entry, and transposed into the bit-sliced planes on the way; the gather
+// kernel then advances st->tvw.  Dead entries are filtered later (their
+// records are flagged by k_rot_gather, which reads every candidate's
+// coefficient anyway).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ planes, u64 cap,
+                                                      u64* __restrict__ tp, u64 tstride, u32 nq,
                                                       const DevState* __restrict__ st, u32 in_slot,
                                                       RotSpec g, u64* __restrict__ masks,
                                                       u64* __restrict__ cnt) {
@@ -147,34 +189,77 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ pl
   if (st->overflow) return;
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
+  const u64 tvw = st->tvw;
   const u32 G = chunk_count(n, kArenaChunk);
-  if (blockIdx.x >= G) return;
-  u64 lo, hi;
-  block_chunk(n, G, blockIdx.x, lo, hi);
-  const u32 nw = (u32)((hi - lo + 63) / 64);
-  const u32 wa = g.a >> 6, wb = g.b >> 6;
-  const bool needx = (g.gaz | g.gbz) != 0u;
-  const bool needz = (g.gax | g.gbx) != 0u;
-  u64 count = 0;
-  for (u32 j = wave; j < nw; j += kWaves * kUnroll) {
-    u32 r[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const u64 i = lo + (u64)(j + u * kWaves) * 64 + lane;
-      r[u] = (i < hi) ? rot_test<W>(planes, cap, i, g, wa, wb, needx, needz) : 0u;
+  const u64* pax = tp + (u64)g.a * tstride;
+  const u64* paz = tp + (u64)(nq + g.a) * tstride;
+  const u64* pbx = tp + (u64)g.b * tstride;
+  const u64* pbz = tp + (u64)(nq + g.b) * tstride;
+  for (u32 ch = blockIdx.x; ch < G; ch += gridDim.x) {
+    u64 lo, hi;
+    block_chunk(n, G, ch, lo, hi);
+    const u64 w0 = lo >> 6, w1 = (hi + 63) >> 6;  // lo is a multiple of 256
+    const u64 wv = tvw < w0 ? w0 : (tvw > w1 ? w1 : tvw);  // [w0, wv): current words
+    u64 count = 0;
+    // current words: entries below 64 tvw <= n are whole words, and chunk
+    // boundaries other than n are multiples of 256, so no word is partial here
+    for (u64 w = w0 + threadIdx.x; w < wv; w += kBlock) {
+      u64 m = 0;
+      if (g.gaz) m ^= pax[w];
+      if (g.gax) m ^= paz[w];
+      if (g.two) {
+        if (g.gbz) m ^= pbx[w];
+        if (g.gbx) m ^= pbz[w];
+      }
+      masks[w] = m;
+      count += (u64)__popcll(m);
     }
+    // remaining words: test from the key planes ...
+    for (u64 w = wv + wave; w < w1; w += kWaves) {
+      const u64 i = w * 64 + lane;
+      const bool in = i < hi;
+      u64 k[2 * W];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const u64 m = __ballot(r[u] != 0u);
-      const u32 ju = j + u * kWaves;
-      if (ju < nw) {
-        if (lane == 0) masks[(lo >> 6) + ju] = m;
+      for (int p = 0; p < 2 * W; ++p) k[p] = 0;
+      if (in) load_key<W>(planes, cap, i, k);
+      const u64 m = __ballot(in && rot_anti<W>(k, g) != 0u);
+      if (lane == 0) {
+        masks[w] = m;
         count += (u64)__popcll(m);
       }
     }
+    const u64 tot = block_sum_all(count, lds);
+    if (threadIdx.x == 0) cnt[ch] = tot;
   }
-  const u64 tot = block_sum_all(lane == 0 ? count : 0ull, lds);  // count is wave-uniform
-  if (threadIdx.x == 0) cnt[blockIdx.x] = tot;
+  // ... and refresh the bit-sliced planes.  The stale words [tvw, ceil(n/64))
+  // are shared out over every wave of the grid, one (group of 8 aligned words
+  // = 512 entries, key plane word) item per wave step, so that every plane
+  // receives whole 64-byte lines; a group that starts below tvw recomputes
+  // current words from the same keys (equal values, so no conflict with
+  // concurrent readers).
+  {
+    const u64 nwords = (n + 63) >> 6;
+    const u64 waves = (u64)gridDim.x * kWaves;
+    const u64 items_lo = (tvw >> 3) * (2 * W), items_hi = ((nwords + 7) >> 3) * (2 * W);
+    for (u64 it = items_lo + (u64)blockIdx.x * kWaves + wave; it < items_hi; it += waves) {
+      const u64 wb = (it / (2 * W)) << 3;  // group of 8 words
+      const int p = (int)(it % (2 * W));   // key plane word
+      u64 r[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const u64 i = (wb + (u64)j) * 64 + lane;
+        r[j] = i < n ? planes[(u64)p * cap + i] : 0ull;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) r[j] = transpose64(r[j], lane);
+      const u32 q = (u32)(p % W) * 64u + lane;
+      if (q < nq) {
+        u64* dst = tp + ((p < W ? 0ull : (u64)nq) + q) * tstride + wb;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) dst[j] = r[j];
+      }
+    }
+  }
 }
 
 // ---- k_rot_gather ---------------------------------------------------------------
@@ -193,29 +278,36 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ p
   __shared__ u32 loff[kChunkMaxWords];
   __shared__ u32 lt[kBlock];
   if (st->overflow) return;
-  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
   const u32 G = chunk_count(n, kArenaChunk);
-  if (blockIdx.x >= G) return;
-  u64 lo, hi;
-  block_chunk(n, G, blockIdx.x, lo, hi);
-  const u32 nw = (u32)((hi - lo + 63) / 64);
-  const u64 base_off = chunk_offset(cnt, blockIdx.x, lds);
   if (blockIdx.x == 0) {
     const u64 tot = chunk_offset(cnt, G, lds);
-    if (threadIdx.x == 0) st->n_wl = tot;
+    if (threadIdx.x == 0) {
+      st->n_wl = tot;
+      // k_rot_count refreshed the bit-sliced planes of every word from the old
+      // tvw on; the last word stays stale while it is partial
+      const u64 nw_full = n >> 6;
+      if (nw_full > st->tvw) st->transposed += n - (st->tvw << 6);
+      st->tvw = nw_full;
+    }
   }
-  chunk_mask_scan(masks, lo >> 6, nw, lm, loff, lt);
-  for (u32 j = wave; j < nw; j += kWaves) {
-    const u64 m = lm[j];
-    if ((m >> lane) & 1ull) {
-      const u64 i = lo + (u64)j * 64 + lane;
+  for (u32 ch = blockIdx.x; ch < G; ch += gridDim.x) {
+    u64 lo, hi;
+    block_chunk(n, G, ch, lo, hi);
+    const u32 nw = (u32)((hi - lo + 63) / 64);
+    const u64 base_off = chunk_offset(cnt, ch, lds);
+    chunk_mask_scan(masks, lo >> 6, nw, lm, loff, lt);
+    // one thread per selected entry (rank order), so every lane has a load in
+    // flight and consecutive threads write consecutive records
+    const u32 total = nw ? loff[nw - 1] + (u32)__popcll(lm[nw - 1]) : 0u;
+    for (u32 r = threadIdx.x; r < total; r += kBlock) {
+      const u64 i = lo + chunk_rank_pos(lm, loff, nw, r);
       u64 k[2 * W];
       load_key<W>(planes, cap, i, k);
       const double c = coef[i];
-      store_rec<W>(wl, base_off + loff[j] + (u64)__popcll(m & lanes_below(lane)), k, c, (u32)i,
-                   is_dead(c) ? 2u : 0u);
+      store_rec<W>(wl, base_off + r, k, c, (u32)i, is_dead(c) ? 2u : 0u);
     }
+    __syncthreads();  // lm / loff are reused by the next chunk
   }
 }
 
@@ -317,42 +409,43 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
   const u32 G = chunk_count(n_wl, kListChunk);
-  if (blockIdx.x >= G) return;
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  u64 lo, hi;
-  block_chunk(n_wl, G, blockIdx.x, lo, hi);
-  const u32 nw = (u32)((hi - lo + 63) / 64);
-  u64 count = 0, kills = 0;
-  for (u32 j = wave; j < nw; j += kWaves) {
-    const u64 w = lo + (u64)j * 64 + lane;
-    bool keep = false;
-    u64 k[2 * W];
-    double c = 0.0;
-    u32 idx = 0, fl = 1u;
-    if (w < hi) load_rec<W>(wl, w, k, c, idx, fl);
-    if (fl == 0u) {
-      const double eps = rot_eps<W>(k, g);
-      const double cp = __dmul_rn(c, g.cs);
-      const double cc = __dmul_rn(c, eps * g.sn);
-      bool dp = false;
-      keep = true;
-      if (g.trunc) {
-        dp = drop_term<W>(k, cp, t);
-        rot_flip<W>(k, g);
-        keep = !drop_term<W>(k, cc, t);
+  for (u32 ch = blockIdx.x; ch < G; ch += gridDim.x) {
+    u64 lo, hi;
+    block_chunk(n_wl, G, ch, lo, hi);
+    const u32 nw = (u32)((hi - lo + 63) / 64);
+    u64 count = 0, kills = 0;
+    for (u32 j = wave; j < nw; j += kWaves) {
+      const u64 w = lo + (u64)j * 64 + lane;
+      bool keep = false;
+      u64 k[2 * W];
+      double c = 0.0;
+      u32 idx = 0, fl = 1u;
+      if (w < hi) load_rec<W>(wl, w, k, c, idx, fl);
+      if (fl == 0u) {
+        const double eps = rot_eps<W>(k, g);
+        const double cp = __dmul_rn(c, g.cs);
+        const double cc = __dmul_rn(c, eps * g.sn);
+        bool dp = false;
+        keep = true;
+        if (g.trunc) {
+          dp = drop_term<W>(k, cp, t);
+          rot_flip<W>(k, g);
+          keep = !drop_term<W>(k, cc, t);
+        }
+        coef[idx] = dp ? dead_value() : cp;
+        kills += (u64)dp;
       }
-      coef[idx] = dp ? dead_value() : cp;
-      kills += (u64)dp;
+      const u64 m = __ballot(keep);
+      if (lane == 0) masks[(lo >> 6) + j] = m;
+      count += (u64)__popcll(m);
     }
-    const u64 m = __ballot(keep);
-    if (lane == 0) masks[(lo >> 6) + j] = m;
-    count += (u64)__popcll(m);
-  }
-  const u64 tc = block_sum_all(lane == 0 ? count : 0ull, lds);  // count is wave-uniform
-  const u64 tk = block_sum_all(kills, lds);
-  if (threadIdx.x == 0) {
-    cnt[blockIdx.x] = tc;
-    kills_out[blockIdx.x] = tk;
+    const u64 tc = block_sum_all(lane == 0 ? count : 0ull, lds);  // count is wave-uniform
+    const u64 tk = block_sum_all(kills, lds);
+    if (threadIdx.x == 0) {
+      cnt[ch] = tc;
+      kills_out[ch] = tk;
+    }
   }
 }
 
@@ -378,22 +471,10 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
   const u32 G = chunk_count(n_wl, kListChunk);
-  if (blockIdx.x >= G) return;
-  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  u64 lo, hi;
-  block_chunk(n_wl, G, blockIdx.x, lo, hi);
-  const u32 nw = (u32)((hi - lo + 63) / 64);
-  const u64 base_off = n_entries + chunk_offset(cnt, blockIdx.x, lds);
   if (blockIdx.x == 0) {
     const u64 appended = chunk_offset(cnt, G, lds);
-    u64 s = 0, sp = 0;
-    for (u32 j = threadIdx.x; j < G; j += kBlock) s += ekills[j];
-    for (u32 j = threadIdx.x; j < Gp; j += kBlock) {
-      s += pkills[j];
-      sp += ppairs[j];
-    }
-    const u64 kills = block_sum_all(s, lds);
-    const u64 pairs = block_sum_all(sp, lds);
+    const u64 kills = block_sum_prefix(ekills, G, lds) + block_sum_prefix(pkills, Gp, lds);
+    const u64 pairs = block_sum_prefix(ppairs, Gp, lds);
     if (threadIdx.x == 0) {
       st->slot[out_slot].n_entries = n_entries + appended;
       st->slot[out_slot].n_dead = in.n_dead + kills;
@@ -401,18 +482,23 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
       st->gates += 1;
       st->rotations += 1;
       st->scanned += n_entries;
-      st->scan_words += n_entries * g.test_words;
+      st->scan_planes += n_entries * g.test_planes;
       st->wl_total += n_wl;
       st->pairs_total += pairs;
       st->children_total += appended;
       st->kills_total += kills;
     }
   }
-  chunk_mask_scan(masks, lo >> 6, nw, lm, loff, lt);
-  for (u32 j = wave; j < nw; j += kWaves) {
-    const u64 m = lm[j];
-    if ((m >> lane) & 1ull) {
-      const u64 w = lo + (u64)j * 64 + lane;
+  for (u32 ch = blockIdx.x; ch < G; ch += gridDim.x) {
+    u64 lo, hi;
+    block_chunk(n_wl, G, ch, lo, hi);
+    const u32 nw = (u32)((hi - lo + 63) / 64);
+    const u64 base_off = n_entries + chunk_offset(cnt, ch, lds);
+    chunk_mask_scan(masks, lo >> 6, nw, lm, loff, lt);
+    // one thread per surviving child (rank order): dense loads, coalesced stores
+    const u32 total = nw ? loff[nw - 1] + (u32)__popcll(lm[nw - 1]) : 0u;
+    for (u32 r = threadIdx.x; r < total; r += kBlock) {
+      const u64 w = lo + chunk_rank_pos(lm, loff, nw, r);
       u64 k[2 * W];
       double c;
       u32 idx, fl;
@@ -420,11 +506,12 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
       const double eps = rot_eps<W>(k, g);
       const double cc = __dmul_rn(c, eps * g.sn);
       rot_flip<W>(k, g);
-      const u64 dst = base_off + loff[j] + (u64)__popcll(m & lanes_below(lane));
+      const u64 dst = base_off + r;
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) planes[(u64)p * cap + dst] = k[p];
       coef[dst] = cc;
     }
+    __syncthreads();  // lm / loff are reused by the next chunk
   }
 }
 

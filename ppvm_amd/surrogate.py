@@ -175,6 +175,7 @@ class SweepReport:
     train_loss: list
     abs_err_net: np.ndarray    # network alone (before fallback), per query point
     fallback_live: bool        # fallbacks propagated live (else charged at recorded times)
+    tradeoff: list             # per disagreement bound: fallbacks, accuracy, speedup
 
 
 def run_config5(grid: np.ndarray, h_values: np.ndarray, t_values: np.ndarray, plain_seconds: np.ndarray,
@@ -223,9 +224,19 @@ def run_config5(grid: np.ndarray, h_values: np.ndarray, t_values: np.ndarray, pl
     err = np.abs(pred - Yall[query_idx])
     q_h = np.unique(hi)
     plain_s = float(sum(plain_seconds[i] for i in q_h))
+    # other disagreement bounds, fallbacks charged at the recorded propagation times
+    tradeoff = []
+    for bound in (1e-2, 5e-3, 2e-3, 1e-3, 5e-4):
+        f = sd > bound
+        hs = np.unique(hi[f])
+        e = np.where(f, 0.0, err_net)  # a fallback point takes the propagator's value
+        cost = infer_s + float(sum(plain_seconds[i] for i in hs))
+        tradeoff.append({"std_bound": bound, "fallback_points": int(f.sum()), "fallback_h": int(len(hs)),
+                         "frac_abs_err_le_1e-3": float(np.mean(e <= 1e-3)), "max_err": float(e.max()),
+                         "speedup_recorded": plain_s / cost})
     surrogate_s = infer_s + fallback_s
     net.close()
     return SweepReport(len(query_idx), int(fb.sum()), len(fb_h), err, float(np.mean(err <= 1e-3)),
                        float(err.max()), train_s, infer_s, fallback_s, surrogate_s, plain_s,
                        plain_s / surrogate_s if surrogate_s > 0 else float("inf"), loss.tolist(),
-                       err_net, propagate is not None)
+                       err_net, propagate is not None, tradeoff)
