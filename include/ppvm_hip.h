@@ -173,6 +173,12 @@ int ppvm_truncate(ppvm_handle h);
  * (crates/ppvm-python-native/src/interface.rs:122-130, sum/trace.rs:19-21):
  * the sum of coefficients of terms whose x plane is all zero. */
 int ppvm_overlap_with_zero(ppvm_handle h, double* out);
+/* Replaces trace(pattern) (crates/ppvm-python-native/src/interface.rs:122-126,
+ * sum/trace.rs:9-22): the sum of coefficients of terms the Pauli pattern accepts,
+ * with the grammar of crates/ppvm-pauli-word/src/pattern/parse.rs:30-143 and
+ * the matcher of pattern/contains.rs:20-151 ("Z?*" = ppvm_overlap_with_zero).
+ * A pattern that does not parse (the reference panics) returns PPVM_EINVAL. */
+int ppvm_trace(ppvm_handle h, const char* pattern, double* out);
 /* Replaces PauliSum::overlap (crates/ppvm-pauli-sum/src/sum/trace.rs:30-37):
  * sum over terms k of b of b[k] * a[k].  Both handles must have equal n_qubits. */
 int ppvm_overlap(ppvm_handle a, ppvm_handle b, double* out);

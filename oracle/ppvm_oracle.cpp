@@ -25,11 +25,13 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cctype>
 #include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <functional>
 #include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 
@@ -436,6 +438,165 @@ double overlap_zero(const Sum& s) {
 }
 
 // ---------------------------------------------------------------------------
+// Pauli patterns for trace(pattern): grammar of
+// crates/ppvm-pauli-word/src/pattern/parse.rs:30-143 and greedy matcher of
+// pattern/contains.rs:20-151.  A single-site pattern is kept as the set of
+// accepted Pauli codes (bit c for code c = x | z << 1: I 0, X 1, Z 2, Y 3).
+// As in contains.rs:39 the "any Pauli or identity" form ('_', '[XYZ]?')
+// accepts the identity only.
+enum PatKind { PK_POSITION = 0, PK_STAR = 1, PK_REPEAT = 2 };
+struct PatItem {
+  int kind;
+  unsigned accept;
+  uint64_t arg;
+};
+constexpr unsigned kAcceptAnyOrI = 1u;  // pattern/contains.rs:39
+
+// usize::parse on a digit string: non-empty, no overflow
+bool parse_usize(const std::string& num, uint64_t& v) {
+  if (num.empty()) return false;
+  v = 0;
+  for (char ch : num) {
+    const uint64_t d = (uint64_t)(ch - '0');
+    if (v > (UINT64_MAX - d) / 10) return false;
+    v = v * 10 + d;
+  }
+  return true;
+}
+
+bool parse_pattern(const char* text, std::vector<PatItem>& out) {
+  std::string str(text);
+  size_t a = 0, b = str.size();
+  while (a < b && std::isspace((unsigned char)str[a])) ++a;  // str::trim
+  while (b > a && std::isspace((unsigned char)str[b - 1])) --b;
+  const std::string in = str.substr(a, b - a);
+  size_t i = 0;
+  auto code_of = [](char c) -> int { return c == 'X' ? 1 : c == 'Z' ? 2 : c == 'Y' ? 3 : -1; };
+  out.clear();
+  while (i < in.size()) {
+    const char c = in[i++];
+    unsigned accept = 0;
+    int nset = 0;  // 1 single, 2 double, 3 any non-identity, 0 any-or-identity
+    if (c == '[') {
+      unsigned set = 0;
+      while (i < in.size()) {
+        const char ch = in[i];
+        if (ch == ']') {
+          ++i;
+          break;
+        }
+        const int cd = code_of(ch);
+        if (cd < 0) return false;  // "Expected X, Y, or Z in []"
+        set |= 1u << cd;
+        ++i;
+      }
+      nset = __builtin_popcount(set);
+      if (nset < 1 || nset > 3) return false;
+      accept = set;
+    } else if (c == '_') {
+      nset = 0;
+      accept = kAcceptAnyOrI;
+    } else {
+      const int cd = code_of(c);
+      if (cd < 0) return false;
+      nset = 1;
+      accept = 1u << cd;
+    }
+    if (i < in.size() && in[i] == '?') {  // OpPattern::add_identity (data.rs:37-45)
+      ++i;
+      if (nset == 1 || nset == 2) accept |= 1u;
+      else if (nset == 3) accept = kAcceptAnyOrI;
+    }
+    PatItem it{0, accept, 0};
+    if (i < in.size() && in[i] == '*') {
+      ++i;
+      it.kind = PK_STAR;
+    } else if (i < in.size() && in[i] == '{') {
+      ++i;
+      std::string num;
+      while (i < in.size()) {
+        const char ch = in[i];
+        if (std::isdigit((unsigned char)ch)) {
+          num.push_back(ch);
+          ++i;
+        } else if (ch == '}') {
+          ++i;
+          break;
+        } else {
+          return false;
+        }
+      }
+      if (!parse_usize(num, it.arg)) return false;
+      it.kind = PK_REPEAT;
+    } else if (i < in.size() && std::isdigit((unsigned char)in[i])) {
+      std::string num;
+      while (i < in.size() && std::isdigit((unsigned char)in[i])) num.push_back(in[i++]);
+      if (!parse_usize(num, it.arg)) return false;
+      it.kind = PK_POSITION;
+    } else {
+      return false;  // "Expected '*' or digit after Pauli pattern"
+    }
+    out.push_back(it);
+  }
+  return true;
+}
+
+bool pattern_contains(const Sum& s, const uint64_t* k, const std::vector<PatItem>& pat) {
+  const uint64_t n = s.n;
+  uint64_t cur = 0;  // next qubit the word iterator yields (word/data.rs:421-428)
+  auto code = [&](uint64_t q) -> unsigned { return (unsigned)s.xb(k, (uint32_t)q) | ((unsigned)s.zb(k, (uint32_t)q) << 1); };
+  for (const PatItem& it : pat) {
+    bool next = false;
+    if (it.kind == PK_POSITION) {  // match_position, contains.rs:48-70
+      while (cur < n) {
+        const uint64_t p = cur++;
+        const unsigned ch = code(p);
+        if (p == it.arg) {
+          next = (it.accept >> ch) & 1u;
+          break;
+        }
+        if (ch != 0) break;  // non-identity before the anchored position
+      }
+    } else if (it.kind == PK_STAR) {  // match_star, contains.rs:72-88
+      while (cur < n && ((it.accept >> code(cur)) & 1u)) ++cur;
+      next = true;
+    } else {  // match_repeat, contains.rs:90-117
+      uint64_t matched = 0;
+      bool decided = false;
+      while (cur < n) {
+        const unsigned ch = code(cur);
+        if ((it.accept >> ch) & 1u) {
+          ++cur;
+          ++matched;
+          if (matched == it.arg) {
+            next = true;
+            decided = true;
+            break;
+          }
+        } else {
+          next = !(matched < it.arg);
+          decided = true;
+          break;
+        }
+      }
+      if (!decided) next = matched == it.arg;
+    }
+    if (!next) return false;
+  }
+  for (; cur < n; ++cur)  // remaining symbols must be identities (contains.rs:138-148)
+    if (code(cur) != 0) return false;
+  return true;
+}
+
+// trace(pattern): fold over the entries in insertion order (map/hashmap.rs:108-127)
+double trace_pattern(const Sum& s, const std::vector<PatItem>& pat) {
+  double acc = 0.0;
+  for (size_t i = 0; i < s.len(); ++i)
+    if (pattern_contains(s, s.key(i), pat)) acc += s.coef[i];
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
 // Sharded multi-thread variant (CPU baseline modelled on the DashMap + rayon
 // configuration, crates/ppvm-traits/src/map/dashmap.rs:42-52,119-133,156-165).
 // Keys are partitioned over shards by hash; a rotation scales parents and
@@ -639,6 +800,24 @@ int ppvo_apply(ppvo_handle h, const ppvm_op* ops, size_t n) {
   return 0;
 }
 double ppvo_overlap_with_zero(ppvo_handle h) { return overlap_zero(*h); }
+int ppvo_trace(ppvo_handle h, const char* pattern, double* out) {
+  std::vector<PatItem> pat;
+  if (!pattern || !parse_pattern(pattern, pat)) return 1;
+  *out = trace_pattern(*h, pat);
+  return 0;
+}
+int ppvo_pattern_contains(ppvo_handle h, const char* pattern, const uint64_t* x, const uint64_t* z,
+                          uint8_t* out, size_t count) {
+  std::vector<PatItem> pat;
+  if (!pattern || !parse_pattern(pattern, pat)) return 1;
+  std::vector<uint64_t> k(h->NW);
+  for (size_t i = 0; i < count; ++i) {
+    std::memcpy(k.data(), x + i * h->W, h->W * 8);
+    std::memcpy(k.data() + h->W, z + i * h->W, h->W * 8);
+    out[i] = pattern_contains(*h, k.data(), pat) ? 1 : 0;
+  }
+  return 0;
+}
 double ppvo_overlap(ppvo_handle a, ppvo_handle b) {
   // sum over b's entries of b[k] * a.trace(k)  (sum/trace.rs:30-37)
   double acc = 0.0;

@@ -37,6 +37,12 @@ void ppvo_add_terms(ppvo_handle h, const uint64_t* x, const uint64_t* z, const d
 int ppvo_apply(ppvo_handle h, const ppvm_op* ops, size_t n);
 double ppvo_overlap_with_zero(ppvo_handle h);
 double ppvo_overlap(ppvo_handle a, ppvo_handle b);
+/* trace(pattern) (pattern grammar of crates/ppvm-pauli-word/src/pattern/parse.rs);
+ * returns nonzero if the pattern does not parse. */
+int ppvo_trace(ppvo_handle h, const char* pattern, double* out);
+/* Pattern membership of `count` words (x, z: count*W words, term-major). */
+int ppvo_pattern_contains(ppvo_handle h, const char* pattern, const uint64_t* x, const uint64_t* z,
+                          uint8_t* out, size_t count);
 uint64_t ppvo_len(ppvo_handle h);
 /* Writes min(cap, len) terms in insertion order; returns len. */
 uint64_t ppvo_export(ppvo_handle h, uint64_t* x, uint64_t* z, double* c, size_t cap);

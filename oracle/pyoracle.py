@@ -50,6 +50,11 @@ def lib():
         L.ppvo_overlap_with_zero.argtypes = [ctypes.c_void_p]
         L.ppvo_overlap.restype = ctypes.c_double
         L.ppvo_overlap.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ppvo_trace.restype = ctypes.c_int
+        L.ppvo_trace.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _f64p]
+        L.ppvo_pattern_contains.restype = ctypes.c_int
+        L.ppvo_pattern_contains.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _u64p, _u64p,
+                                            ctypes.c_void_p, ctypes.c_size_t]
         L.ppvo_len.restype = ctypes.c_uint64
         L.ppvo_len.argtypes = [ctypes.c_void_p]
         L.ppvo_export.restype = ctypes.c_uint64
@@ -134,6 +139,23 @@ class OracleSum:
 
     def overlap(self, other: "OracleSum") -> float:
         return lib().ppvo_overlap(self._h, other._h)
+
+    def trace(self, pattern: str) -> float:
+        """Sum of the coefficients of terms matching a Pauli pattern
+        (crates/ppvm-pauli-word/src/pattern/); ValueError if it does not parse."""
+        out = np.zeros(1, dtype=np.float64)
+        if lib().ppvo_trace(self._h, pattern.encode(), out.ctypes.data_as(_f64p)) != 0:
+            raise ValueError(f"invalid Pauli pattern {pattern!r}")
+        return float(out[0])
+
+    def pattern_contains(self, pattern: str, strings) -> list:
+        """Pattern membership of each Pauli string (on this sum's qubit count)."""
+        x, z = pack_strings(list(strings), self.n_qubits)
+        out = np.zeros(len(strings), dtype=np.uint8)
+        if lib().ppvo_pattern_contains(self._h, pattern.encode(), _p64(x), _p64(z), out.ctypes.data,
+                                       len(strings)) != 0:
+            raise ValueError(f"invalid Pauli pattern {pattern!r}")
+        return [bool(v) for v in out]
 
     def __len__(self) -> int:
         return int(lib().ppvo_len(self._h))

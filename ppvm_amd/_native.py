@@ -69,6 +69,7 @@ def _declare(L):
     d("ppvm_truncate", _H)
     d("ppvm_overlap_with_zero", _H, _f64p)
     d("ppvm_overlap", _H, _H, _f64p)
+    d("ppvm_trace", _H, ctypes.c_char_p, _f64p)
     d("ppvm_len", _H, _u64p)
     d("ppvm_export_terms", _H, _u64p, _u64p, _f64p, ctypes.c_size_t,
       ctypes.POINTER(ctypes.c_size_t))
@@ -195,6 +196,11 @@ class Handle:
     def overlap_with_zero(self) -> float:
         v = ctypes.c_double()
         check(lib().ppvm_overlap_with_zero(self.h, ctypes.byref(v)))
+        return v.value
+
+    def trace(self, pattern: str) -> float:
+        v = ctypes.c_double()
+        check(lib().ppvm_trace(self.h, pattern.encode(), ctypes.byref(v)))
         return v.value
 
     def __len__(self) -> int:

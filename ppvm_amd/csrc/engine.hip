@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -392,6 +393,23 @@ int launch_overlap(ppvm_sum* s) {
   return PPVM_OK;
 }
 
+template <int W>
+int launch_trace(ppvm_sum* s, const PatternProg& prog) {
+  int rc;
+  const u32 in = (u32)(s->op_seq & 1);
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_trace<W>, dim3(kOverlapGrid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
+                     s->coef[s->cur], s->cap, s->st, in, prog, s->partials);
+  if ((rc = prof_end(s))) return rc;
+  FinishArgs fa{};
+  fa.mode = F_OVERLAP;
+  fa.Ga = kOverlapGrid;
+  fa.dpart = s->partials;
+  if ((rc = launch_finish(s, fa))) return rc;
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
 // Σ_{k in b} c_b(k) c_a(k): a's live keys into a's table, then one probe per
 // live term of b, all on a's stream (b is synchronised by the caller).
 template <int W>
@@ -452,6 +470,82 @@ int issue_map_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_map, 
 int overlap_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_overlap); }
 int compact_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_compact); }
 int overlap2_w(ppvm_sum* s, ppvm_sum* b) { PPVM_DISPATCH_W(s, launch_overlap2, b); }
+int trace_w(ppvm_sum* s, const PatternProg& prog) { PPVM_DISPATCH_W(s, launch_trace, prog); }
+
+// Pauli pattern text -> matcher items (crates/ppvm-pauli-word/src/pattern/parse.rs:30-143).
+// Symbol sets are kept as accepted Pauli codes; '?' adds the identity to one or
+// two letters, and turns the three-letter set into the "any Pauli or identity"
+// form, which (like '_') accepts the identity only (pattern/contains.rs:39).
+int compile_pattern(const char* text, u32 nq, PatternProg& prog) {
+  std::memset(&prog, 0, sizeof(prog));
+  prog.nq = nq;
+  std::string in(text);
+  while (!in.empty() && std::isspace((unsigned char)in.back())) in.pop_back();
+  size_t i = 0;
+  while (i < in.size() && std::isspace((unsigned char)in[i])) ++i;
+  auto letter = [](char c) -> int { return c == 'X' ? 1 : c == 'Z' ? 2 : c == 'Y' ? 3 : -1; };
+  auto bad = [&](const char* why) { return fail(PPVM_EINVAL, "invalid Pauli pattern '%s': %s", text, why); };
+  auto number = [&](u64& v) -> bool {  // usize::parse: digits only, no overflow
+    size_t d = 0;
+    v = 0;
+    while (i < in.size() && std::isdigit((unsigned char)in[i])) {
+      const u64 digit = (u64)(in[i++] - '0');
+      if (v > (~0ull - digit) / 10) return false;
+      v = v * 10 + digit;
+      ++d;
+    }
+    return d > 0;
+  };
+  while (i < in.size()) {
+    const char c = in[i++];
+    u32 set = 0;
+    if (c == '[') {
+      while (i < in.size() && in[i] != ']') {
+        const int l = letter(in[i++]);
+        if (l < 0) return bad("expected X, Y or Z inside []");
+        set |= 1u << l;
+      }
+      if (i < in.size()) ++i;  // ']'
+      if (set == 0) return bad("empty []");
+    } else if (c == '_') {
+      set = 0;
+    } else {
+      const int l = letter(c);
+      if (l < 0) return bad("expected X, Y, Z, _ or [");
+      set = 1u << l;
+    }
+    const int letters = __builtin_popcount(set);
+    u32 accept = letters ? set : 1u;
+    if (i < in.size() && in[i] == '?') {
+      ++i;
+      accept = (letters == 1 || letters == 2) ? (set | 1u) : 1u;
+    }
+    if (prog.n_items >= (u32)kMaxPatItems) return bad("too many items");
+    const u32 t = prog.n_items;
+    prog.accept[t] = accept;
+    if (i < in.size() && in[i] == '*') {
+      ++i;
+      prog.kind[t] = PK_STAR;
+    } else if (i < in.size() && in[i] == '{') {
+      ++i;
+      u64 v;
+      if (!number(v)) return bad("expected a count after '{'");
+      if (i < in.size()) {
+        if (in[i] != '}') return bad("expected '}'");
+        ++i;
+      }
+      prog.kind[t] = PK_REPEAT;
+      prog.arg[t] = v;
+    } else {
+      u64 v;
+      if (!number(v)) return bad("expected '*', '{' or a qubit index after a symbol");
+      prog.kind[t] = PK_POSITION;
+      prog.arg[t] = v;
+    }
+    prog.n_items = t + 1;
+  }
+  return PPVM_OK;
+}
 
 int issue(ppvm_sum* s, Issued& it, bool log_it) {
   int rc = it.kind == IK_ROT ? issue_rot_w(s, it) : issue_map_w(s, it);
@@ -1123,6 +1217,18 @@ int ppvm_overlap_with_zero(ppvm_handle h, double* out) {
   int rc;
   if ((rc = ppvm_synchronize(h))) return rc;
   if ((rc = overlap_w(h))) return rc;
+  if ((rc = read_state(h))) return rc;
+  *out = h->st_host->overlap;
+  return PPVM_OK;
+}
+
+int ppvm_trace(ppvm_handle h, const char* pattern, double* out) {
+  if (!h || !pattern || !out) return fail(PPVM_EINVAL, "null argument");
+  PatternProg prog;
+  int rc;
+  if ((rc = compile_pattern(pattern, h->n, prog))) return rc;
+  if ((rc = ppvm_synchronize(h))) return rc;
+  if ((rc = trace_w(h, prog))) return rc;
   if ((rc = read_state(h))) return rc;
   *out = h->st_host->overlap;
   return PPVM_OK;

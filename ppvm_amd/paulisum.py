@@ -189,9 +189,13 @@ class PauliSum:
         return self._h.overlap_with_zero()
 
     def trace(self, pattern: str) -> float:
-        if pattern != "Z?*":
-            raise NotImplementedError("only the 'Z?*' pattern (overlap_with_zero) is supported")
-        return self.overlap_with_zero()
+        """Sum of the coefficients of terms matching a Pauli pattern
+        (crates/ppvm-python-native/src/interface.rs:122-126): literal X/Y/Z,
+        '_', [..] sets, '?' (identity allowed), '*', '{n}' and qubit anchors.
+        ValueError if the pattern does not parse."""
+        if pattern.strip() == "Z?*":
+            return self.overlap_with_zero()
+        return self._h.trace(pattern)
 
     def overlap(self, other: "PauliSum") -> float:
         """sum_k other[k] * self[k] (crates/ppvm-pauli-sum/src/sum/trace.rs:30-37)."""

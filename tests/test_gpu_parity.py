@@ -201,3 +201,31 @@ def test_overlap_between_sums(n):
     rs[0].apply(O.Program().add(O.H, 0).array())
     og, oo = gs[0].overlap(gs[1]), rs[0].overlap(rs[1])
     assert abs(og - oo) <= 1e-12 * max(1.0, abs(oo))
+
+
+PATTERNS = ["Z?*", "X?*", "[XY]?*", "[XZ]*", "Z0", "Z?0Z?1X2", "[XY]?{2}Z2", "X{2}", "X{0}", "_*",
+            "[XYZ]*", "Z?*X5", "Y3Z?*", "[YZ]?*Z?*", ""]
+
+
+@pytest.mark.parametrize("n", [6, 70, 130])
+def test_trace_patterns(n):
+    """trace(pattern) on the GPU vs the oracle after a truncated random circuit
+    (relative 1e-12: the GPU sums in another association)."""
+    rng = np.random.default_rng(900 + n)
+    strings = random_terms(rng, n, 60)
+    # add low-support words so that anchored / star patterns have matches
+    for q in range(min(n, 8)):
+        strings.append("I" * q + "Z" + "I" * (n - q - 1))
+        strings.append("I" * q + "X" + "I" * (n - q - 1))
+    strings = list(dict.fromkeys(strings))
+    g, r = _pair(n, strings, rng.normal(size=len(strings)).tolist(), tau=1e-4)
+    prog = random_program(rng, n, 30, trunc_every=True, kinds=["h", "s", "cnot", "rx", "rz", "rzz"])
+    g.apply(prog)
+    r.apply(prog)
+    assert len(g) == len(r)
+    for pat in PATTERNS:
+        tg, tr = g.trace(pat), r.trace(pat)
+        assert abs(tg - tr) <= 1e-12 * max(1.0, abs(tr)), (pat, tg, tr)
+    for bad in ("A", "[]", "X*Y"):
+        with pytest.raises(ValueError):
+            g.trace(bad)

@@ -379,3 +379,70 @@ def test_overlap_is_hilbert_schmidt_product():
         hs = np.trace(A @ B).real / 2**n
         assert abs(sums[0].overlap(sums[1]) - hs) < 1e-12
         assert abs(sums[1].overlap(sums[0]) - hs) < 1e-12
+
+
+# ---- trace(pattern) ----------------------------------------------------------------------------
+def _contains(pattern, word):
+    return OracleSum(len(word), 0.0, None).pattern_contains(pattern, [word])[0]
+
+
+def test_pattern_reference_vectors():
+    """crates/ppvm-pauli-word/src/pattern/contains.rs:165-213 and parse.rs docs."""
+    assert _contains("X0Y1Z2", "XYZ")
+    assert _contains("X?0Y1Z2", "XYZ") and _contains("X?0Y1Z2", "IYZ")
+    assert _contains("[XY]0Y1Z2", "XYZ") and _contains("[XY]0Y1Z2", "YYZ")
+    for w in ("XYZ", "YYZ", "IYZ"):
+        assert _contains("[XY]?0Y1Z2", w)
+    for w in ("XYX", "YYX", "IYX"):
+        assert _contains("[XY]?*", w)
+    for w in ("XYZ", "YYZ", "IYZ"):
+        assert _contains("[XY]?{2}Z2", w)
+    assert not _contains("Z?*", "XYY")
+
+
+def test_pattern_semantics_edges():
+    # anchored positions need identities before them and a match at the anchor
+    assert _contains("Z3", "IIIZ") and not _contains("Z3", "IXIZ") and not _contains("Z3", "IIIX")
+    assert not _contains("Z5", "IIIZ")              # anchor past the word: no match
+    assert _contains("Z1Z3", "IZIZ") and not _contains("Z3Z1", "IZIZ")  # anchors consume in order
+    # trailing symbols must be identities
+    assert not _contains("X0", "XZ") and _contains("X0", "XI")
+    # empty pattern: the identity word only
+    assert _contains("", "III") and not _contains("", "IXI")
+    # the "any Pauli or identity" forms accept the identity only (contains.rs:39)
+    assert _contains("_*", "III") and not _contains("_*", "IXI")
+    assert _contains("[XYZ]?*", "III") and not _contains("[XYZ]?*", "ZII")
+    assert _contains("[XYZ]*", "XYZ") and not _contains("[XYZ]*", "XIZ")
+    # counted repetition: exactly `count` matches consumed, the rest must be identity
+    assert _contains("X{2}", "XXI") and not _contains("X{2}", "XII") and not _contains("X{2}", "XXX")
+    # count 0 (contains.rs:94-116): matches are consumed past the count; a later
+    # non-match ends the item successfully, running out of symbols does not
+    assert _contains("X{0}", "III") and _contains("X{0}", "XII") and not _contains("X{0}", "XXX")
+    # whitespace at the ends is trimmed
+    assert _contains("  Z?*  ", "ZIZ")
+
+
+@pytest.mark.parametrize("bad", ["A", "[]", "X", "[XW]0", "X{a}", "X*Y", "X 0", "[XY"])
+def test_pattern_parse_errors(bad):
+    with pytest.raises(ValueError):
+        OracleSum(4, 0.0, None).trace(bad)
+
+
+def test_trace_closed_forms():
+    rng = np.random.default_rng(21)
+    n = 6
+    strs = list({"".join("IXYZ"[v] for v in rng.integers(0, 4, n)) for _ in range(300)}) + ["I" * n]
+    coefs = rng.normal(size=len(strs))
+    s = _sum(n, list(zip(strs, coefs.tolist())))
+    d = dict(zip(strs, coefs.tolist()))
+    assert s.trace("Z?*") == s.overlap_with_zero()
+    assert s.trace("") == d["I" * n]
+    assert s.trace("_*") == d["I" * n]
+    for w in strs[:40]:  # a fully anchored pattern selects exactly one word
+        assert s.trace("".join(f"{ch}{q}" for q, ch in enumerate(w) if ch != "I")) == d[w]
+    # X?* sums the words made of I and X only, in insertion order
+    want = 0.0
+    for w, c in zip(strs, coefs.tolist()):
+        if set(w) <= {"I", "X"}:
+            want += c
+    assert s.trace("X?*") == want
