@@ -68,13 +68,16 @@ enum ppvm_opcode {
   /* diagonal noise channels, crates/ppvm-pauli-sum/src/sum/noise.rs:17-131 */
   PPVM_OP_PAULI_ERROR = 40, PPVM_OP_DEPOLARIZE1 = 41, PPVM_OP_DEPOLARIZE2 = 42,
   PPVM_OP_PAULI_ERROR2 = 43,
+  /* amplitude damping (gamma in p[0]), crates/ppvm-pauli-sum/src/sum/noise.rs:132-157 */
+  PPVM_OP_AMPLITUDE_DAMPING = 44,
   /* explicit truncation, crates/ppvm-pauli-sum/src/sum/data.rs:275 */
   PPVM_OP_TRUNCATE = 60
 };
 
 /* One operation of a program.  q1 is the second qubit of two-qubit ops;
  * theta the rotation angle; p[] the channel probabilities (3 for
- * PAULI_ERROR, 1 for DEPOLARIZE1/2, 15 for PAULI_ERROR2). */
+ * PAULI_ERROR, 1 for DEPOLARIZE1/2, 15 for PAULI_ERROR2, gamma for
+ * AMPLITUDE_DAMPING). */
 typedef struct ppvm_op {
   uint32_t code;
   uint32_t q0;
@@ -165,6 +168,11 @@ int ppvm_pauli_error(ppvm_handle h, uint32_t q, const double p[3]);
 int ppvm_depolarize1(ppvm_handle h, uint32_t q, double p);
 int ppvm_depolarize2(ppvm_handle h, uint32_t a, uint32_t b, double p);
 int ppvm_two_qubit_pauli_error(ppvm_handle h, uint32_t a, uint32_t b, const double p[15]);
+/* Replaces AmplitudeDamping::amplitude_damping (crates/ppvm-pauli-sum/src/sum/noise.rs:132-157,
+ * crates/ppvm-python-native/src/interface.rs:365-368): X, Y on q scale by
+ * sqrt(1 - gamma); Z scales by (1 - gamma) and branches gamma * c onto the word
+ * with I on q (merged with += like a rotation child). */
+int ppvm_amplitude_damping(ppvm_handle h, uint32_t q, double gamma);
 
 /* Replaces PauliSum::truncate (crates/ppvm-pauli-sum/src/sum/data.rs:275-311). */
 int ppvm_truncate(ppvm_handle h);

@@ -283,7 +283,11 @@ inline int rot_child(const Sum& s, uint32_t code, uint32_t a, uint32_t b, const 
 // PauliSum::map_insert (sum/data.rs:204-226): parents scaled in place while
 // children go to the scratch vector, then each child is merged with `+=`.
 void rotate(Sum& s, uint32_t code, uint32_t a, uint32_t b, double theta) {
-  const double sn = std::sin(theta), cs = std::cos(theta);  // Float::sin_cos
+  // Float::sin_cos (coefficient.rs:55-57): one libm sincos call, the combined
+  // form LLVM emits for f64::sin_cos on Linux (glibc's separate sin() can
+  // differ from it in the last bit)
+  double sn, cs;
+  sincos(theta, &sn, &cs);
   s.sk.clear();
   s.sc.clear();
   std::vector<uint64_t> child(s.NW);
@@ -297,6 +301,31 @@ void rotate(Sum& s, uint32_t code, uint32_t a, uint32_t b, double theta) {
     coeff *= (double)eps * sn;  // sin.mul_sign(eps)  (coefficient.rs:47-49)
     s.sk.insert(s.sk.end(), child.begin(), child.end());
     s.sc.push_back(coeff);
+  }
+  for (size_t j = 0; j < s.sc.size(); ++j) s.add_assign(&s.sk[j * s.NW], s.sc[j]);
+}
+
+// AmplitudeDamping for PauliSum (sum/noise.rs:132-157), also a map_insert:
+// X and Y on the qubit scale by sqrt(1 - gamma); Z scales by (1 - gamma) and
+// branches gamma * c onto the word with I on the qubit.
+void amplitude_damping(Sum& s, uint32_t a, double gamma) {
+  s.sk.clear();
+  s.sc.clear();
+  const size_t n = s.len();
+  s.term_updates += n;
+  const double decay = std::sqrt(1.0 - gamma), keep = 1.0 - gamma;
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t* k = s.key(i);
+    const bool x = s.xb(k, a), z = s.zb(k, a);
+    if (x) {
+      s.coef[i] *= decay;
+    } else if (z) {
+      const double child = s.coef[i] * gamma;
+      s.sk.insert(s.sk.end(), k, k + s.NW);
+      s.sk[s.sk.size() - s.NW + s.W + (a >> 6)] &= ~(1ull << (a & 63));  // Z -> I
+      s.sc.push_back(child);
+      s.coef[i] *= keep;
+    }
   }
   for (size_t j = 0; j < s.sc.size(); ++j) s.add_assign(&s.sk[j * s.NW], s.sc[j]);
 }
@@ -372,6 +401,10 @@ int apply_one(Sum& s, const ppvm_op& op) {
     case PPVM_OP_RXX: case PPVM_OP_RYY: case PPVM_OP_RZZ:
       if (!qok(op.q0) || !qok(op.q1) || op.q0 == op.q1) return 1;
       rotate(s, code, op.q0, op.q1, op.theta);
+      return 0;
+    case PPVM_OP_AMPLITUDE_DAMPING:  // noise.rs:132-157, gamma = p[0]
+      if (!qok(op.q0)) return 1;
+      amplitude_damping(s, op.q0, op.p[0]);
       return 0;
     case PPVM_OP_PAULI_ERROR: {  // noise.rs:17-39
       if (!qok(op.q0)) return 1;
@@ -695,7 +728,8 @@ int apply_sharded(Sum& src, const ppvm_op* ops, size_t nops, int threads) {
     bool is_moving = (op.code >= PPVM_OP_H && op.code <= PPVM_OP_CY);
     if (op.code != PPVM_OP_TRUNCATE && op.code != PPVM_OP_NOP) tu += live;
     if (is_rot) {
-      const double sn = std::sin(op.theta), cs = std::cos(op.theta);
+      double sn, cs;
+      sincos(op.theta, &sn, &cs);
       par([&](int t) {
         Sum& m = sh[t].m;
         std::vector<uint64_t> child(m.NW);

@@ -64,6 +64,7 @@ def _declare(L):
     d("ppvm_r", _H, ctypes.c_uint32, ctypes.c_double, ctypes.c_double)
     d("ppvm_pauli_error", _H, ctypes.c_uint32, _f64p)
     d("ppvm_depolarize1", _H, ctypes.c_uint32, ctypes.c_double)
+    d("ppvm_amplitude_damping", _H, ctypes.c_uint32, ctypes.c_double)
     d("ppvm_depolarize2", _H, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_double)
     d("ppvm_two_qubit_pauli_error", _H, ctypes.c_uint32, ctypes.c_uint32, _f64p)
     d("ppvm_truncate", _H)

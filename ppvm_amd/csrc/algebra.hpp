@@ -77,11 +77,12 @@ __device__ __forceinline__ void put_z(u64 (&k)[2 * W], u32 q, u32 v) {
 // Anticommutation of P with G (symplectic product parity on the gate qubits).
 template <int W>
 __device__ __forceinline__ u32 rot_anti(const u64 (&k)[2 * W], const RotSpec& g) {
+  if (g.mode == kBranchDamping) return xbit<W>(k, g.a) ^ 1u;  // I or Z on a
   u32 r = (xbit<W>(k, g.a) & g.gaz) ^ (zbit<W>(k, g.a) & g.gax);
   if (g.two) r ^= (xbit<W>(k, g.b) & g.gbz) ^ (zbit<W>(k, g.b) & g.gbx);
   return r;
 }
-// Child key P xor G.
+// Child key P xor G (for damping: Z -> I on a).
 template <int W>
 __device__ __forceinline__ void rot_flip(u64 (&k)[2 * W], const RotSpec& g) {
   if (g.gax) put_x<W>(k, g.a, xbit<W>(k, g.a) ^ 1u);
@@ -99,6 +100,7 @@ __device__ __forceinline__ void rot_flip(u64 (&k)[2 * W], const RotSpec& g) {
 __device__ __forceinline__ u32 pauli_rank(u32 x, u32 z) { return x ? (z ? 1u : 0u) : 2u; }  // X0 Y1 Z2
 template <int W>
 __device__ __forceinline__ double rot_eps(const u64 (&k)[2 * W], const RotSpec& g) {
+  if (g.mode == kBranchDamping) return 1.0;
   u32 xa = xbit<W>(k, g.a), za = zbit<W>(k, g.a);
   u32 anti_a = (xa & g.gaz) ^ (za & g.gax);
   u32 px, pz, gx, gz;

@@ -602,6 +602,8 @@ int ppvm_batch_create(const ppvm_batch_instance* inst, size_t n, ppvm_batch_hand
                        op.code == PPVM_OP_RXX || op.code == PPVM_OP_RYY || op.code == PPVM_OP_RZZ ||
                        op.code == PPVM_OP_DEPOLARIZE2 || op.code == PPVM_OP_PAULI_ERROR2;
       if (op.code == PPVM_OP_NOP || op.code == PPVM_OP_TRUNCATE) continue;
+      if (op.code == PPVM_OP_AMPLITUDE_DAMPING)
+        return bfail(PPVM_EINVAL, "instance %zu op %u: amplitude damping is not supported in batched mode", i, j);
       if (!is_rot(op.code) && !is_map(op.code)) return bfail(PPVM_EINVAL, "instance %zu op %u: unknown code %u", i, j, op.code);
       if (op.q0 >= s.n_qubits || (two && (op.q1 >= s.n_qubits || op.q1 == op.q0)))
         return bfail(PPVM_EINVAL, "instance %zu op %u: qubit out of range", i, j);
@@ -638,8 +640,7 @@ int ppvm_batch_create(const ppvm_batch_instance* inst, size_t n, ppvm_batch_hand
       d.a = op.q0;
       d.b = op.q1;
       if (is_rot(op.code)) {
-        d.sn = std::sin(op.theta);
-        d.cs = std::cos(op.theta);
+        sincos(op.theta, &d.sn, &d.cs);  // as the single-sum engine (one libm sincos)
       } else if (op.code >= PPVM_OP_PAULI_ERROR && op.code <= PPVM_OP_PAULI_ERROR2) {
         double f[16];
         u32 mask, bq;

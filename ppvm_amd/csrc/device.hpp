@@ -70,13 +70,18 @@ struct alignas(64) DevState {
   u64 pad[5];
 };
 
-// Rotation gate exp(-i theta/2 G) with G acting on qubits a (and b).
+// Branching op: a rotation exp(-i theta/2 G) with G acting on qubits a (and
+// b), or the Z -> (1 - gamma) Z + gamma I branch of amplitude damping on qubit a
+// (then G = Z_a, cs = 1 - gamma, sn = gamma, and the worklist holds the terms
+// with I or Z on a, paired as {I, Z} versions of one word).
+constexpr u32 kBranchRotation = 0, kBranchDamping = 1;
 struct RotSpec {
   u32 a, b;
   u32 two;                 // 1 for a two-qubit generator
   u32 gax, gaz, gbx, gbz;  // generator Pauli bits on a and b
   u32 trunc;               // fused truncation after the gate
   u32 test_planes;         // bit-sliced planes the scan reads (x/z of a, b)
+  u32 mode;                // kBranchRotation, or kBranchDamping (amplitude damping on a)
   double sn, cs;           // sin(theta), cos(theta) from the host libm
 };
 

@@ -68,13 +68,18 @@ u64 pow2_at_least(u64 v) {
 }
 
 bool is_rotation(u32 c) { return c >= PPVM_OP_RX && c <= PPVM_OP_RZZ; }
+// Amplitude damping is queued as two internal ops: this branch on the I/Z
+// terms of the qubit, then the sqrt(1 - gamma) scaling of its X/Y terms (a map op
+// under the public code, which also carries the truncation that follows).
+constexpr u32 kOpDampBranch = 1044;
+bool is_branch(u32 c) { return is_rotation(c) || c == kOpDampBranch; }
 bool is_two_qubit(u32 c) {
   return c == PPVM_OP_CNOT || c == PPVM_OP_CZ || c == PPVM_OP_CY || c == PPVM_OP_RXX ||
          c == PPVM_OP_RYY || c == PPVM_OP_RZZ || c == PPVM_OP_DEPOLARIZE2 ||
          c == PPVM_OP_PAULI_ERROR2;
 }
 bool is_map_op(u32 c) {
-  return (c >= PPVM_OP_X && c <= PPVM_OP_CY) || (c >= PPVM_OP_PAULI_ERROR && c <= PPVM_OP_PAULI_ERROR2);
+  return (c >= PPVM_OP_X && c <= PPVM_OP_CY) || (c >= PPVM_OP_PAULI_ERROR && c <= PPVM_OP_AMPLITUDE_DAMPING);
 }
 // ops after which a truncated state stays truncated (sign flips only)
 bool keeps_clean(u32 c) { return c == PPVM_OP_X || c == PPVM_OP_Y || c == PPVM_OP_Z; }
@@ -637,6 +642,16 @@ int sync_check(ppvm_sum* s) {
 // ---- op translation -----------------------------------------------------------
 RotSpec make_rot(const ppvm_op& op, bool trunc) {
   RotSpec g{};
+  if (op.code == kOpDampBranch) {  // Z -> (1 - gamma) Z + gamma I on q0 (noise.rs:146-155)
+    g.a = g.b = op.q0;
+    g.gaz = 1;
+    g.mode = kBranchDamping;
+    g.trunc = trunc ? 1u : 0u;
+    g.test_planes = 1;
+    g.cs = 1.0 - op.p[0];
+    g.sn = op.p[0];
+    return g;
+  }
   g.a = op.q0;
   g.b = op.q1;
   g.two = 0;
@@ -652,16 +667,17 @@ RotSpec make_rot(const ppvm_op& op, bool trunc) {
   g.trunc = trunc ? 1u : 0u;
   g.test_planes = g.gaz + g.gax + (g.two ? g.gbz + g.gbx : 0u);
   // Float::sin_cos of the reference (crates/ppvm-traits/src/traits/coefficient.rs:55-57):
-  // the host libm, once per gate.
-  g.sn = std::sin(op.theta);
-  g.cs = std::cos(op.theta);
+  // one libm sincos per gate, the combined call LLVM makes for f64::sin_cos on
+  // Linux.  glibc's separate sin() can differ from it in the last bit, so the
+  // oracle and the batched path call sincos too.
+  sincos(op.theta, &g.sn, &g.cs);
   return g;
 }
 
 // Adds one map op (and its factor table) to prog; returns false if full.
 bool map_add(MapProgram& p, u32& ntables, const ppvm_op& op, bool trunc) {
   if (p.n_ops >= (u32)kMaxMapOps) return false;
-  const bool noise = op.code >= PPVM_OP_PAULI_ERROR && op.code <= PPVM_OP_PAULI_ERROR2;
+  const bool noise = op.code >= PPVM_OP_PAULI_ERROR && op.code <= PPVM_OP_AMPLITUDE_DAMPING;
   if (noise && ntables >= (u32)kMaxFactorTables) return false;
   MapOp m{};
   m.a = op.q0;
@@ -675,7 +691,13 @@ bool map_add(MapProgram& p, u32& ntables, const ppvm_op& op, bool trunc) {
     double* f = p.factors[table];
     u32 mask = 0;
     for (int i = 0; i < 16; ++i) f[i] = 1.0;
-    if (op.code == PPVM_OP_PAULI_ERROR) {  // noise.rs:17-39
+    if (op.code == PPVM_OP_AMPLITUDE_DAMPING) {  // X, Y: *= sqrt(1 - gamma), noise.rs:141-144
+      const double decay = std::sqrt(1.0 - op.p[0]);
+      f[1 * 4] = decay;  // code X = 1
+      f[3 * 4] = decay;  // code Y = 3
+      mask = (1u << 4) | (1u << 12);
+      m.b = 0xffffffffu;
+    } else if (op.code == PPVM_OP_PAULI_ERROR) {  // noise.rs:17-39
       const double one = 1.0;
       const double xf = one - op.p[1] * 2.0 - op.p[2] * 2.0;
       const double zf = one - op.p[0] * 2.0 - op.p[1] * 2.0;
@@ -738,7 +760,7 @@ int pump(ppvm_sum* s, bool final) {
       ++i;
       continue;
     }
-    if (is_rotation(op.code)) {
+    if (is_branch(op.code)) {
       if (i + 1 >= q.size() && !final) break;
       const bool trunc_next = i + 1 < q.size() && q[i + 1].code == PPVM_OP_TRUNCATE;
       const bool fused = trunc_next && s->clean && s->trunc_active;
@@ -757,7 +779,7 @@ int pump(ppvm_sum* s, bool final) {
     size_t j = i;
     bool ends_queue = true;
     while (j < q.size()) {
-      if (is_rotation(q[j].code)) {
+      if (is_branch(q[j].code)) {
         ends_queue = false;
         break;
       }
@@ -810,6 +832,11 @@ int pump(ppvm_sum* s, bool final) {
 }
 
 int enqueue(ppvm_sum* s, const ppvm_op& op) {
+  if (op.code == PPVM_OP_AMPLITUDE_DAMPING) {
+    ppvm_op br = op;
+    br.code = kOpDampBranch;
+    s->pending.push_back(br);
+  }
   s->pending.push_back(op);
   if (s->pending.size() >= 4 * (size_t)kMaxMapOps) return pump(s, false);
   return PPVM_OK;
@@ -826,6 +853,8 @@ int validate(const ppvm_sum* s, const ppvm_op& op) {
     if (op.q1 == op.q0) return fail(PPVM_EINVAL, "two-qubit op on identical qubits (%u, %u)", op.q0, op.q1);
   }
   if (is_rotation(c) && !std::isfinite(op.theta)) return fail(PPVM_EINVAL, "rotation angle is not finite");
+  if (c == PPVM_OP_AMPLITUDE_DAMPING && !std::isfinite(op.p[0]))
+    return fail(PPVM_EINVAL, "damping rate is not finite");
   return PPVM_OK;
 }
 
@@ -1173,6 +1202,11 @@ int ppvm_depolarize2(ppvm_handle h, uint32_t a, uint32_t b, double p) {
 int ppvm_two_qubit_pauli_error(ppvm_handle h, uint32_t a, uint32_t b, const double p[15]) {
   if (!h || !p) return fail(PPVM_EINVAL, "null argument");
   return push_gate(h, PPVM_OP_PAULI_ERROR2, a, b, 0.0, p, 15);
+}
+
+int ppvm_amplitude_damping(ppvm_handle h, uint32_t q, double gamma) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  return push_gate(h, PPVM_OP_AMPLITUDE_DAMPING, q, 0, 0.0, &gamma, 1);
 }
 
 int ppvm_truncate(ppvm_handle h) {

@@ -205,11 +205,15 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ pl
     // boundaries other than n are multiples of 256, so no word is partial here
     for (u64 w = w0 + threadIdx.x; w < wv; w += kBlock) {
       u64 m = 0;
-      if (g.gaz) m ^= pax[w];
-      if (g.gax) m ^= paz[w];
-      if (g.two) {
-        if (g.gbz) m ^= pbx[w];
-        if (g.gbx) m ^= pbz[w];
+      if (g.mode == kBranchDamping) {
+        m = ~pax[w];  // I or Z on a
+      } else {
+        if (g.gaz) m ^= pax[w];
+        if (g.gax) m ^= paz[w];
+        if (g.two) {
+          if (g.gbz) m ^= pbx[w];
+          if (g.gbx) m ^= pbz[w];
+        }
       }
       masks[w] = m;
       count += (u64)__popcll(m);
@@ -367,9 +371,19 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
         for (int p = 0; p < 2 * W; ++p) same &= (repj[p] == rep[p]);
         if (same) {
           // second arrival: merge both members
-          const double ei = rot_eps<W>(k, g), ej = rot_eps<W>(kj, g);
-          const double ni = __dadd_rn(__dmul_rn(ci, g.cs), __dmul_rn(cj, ej * g.sn));
-          const double nj = __dadd_rn(__dmul_rn(cj, g.cs), __dmul_rn(ci, ei * g.sn));
+          double ni, nj;
+          if (g.mode == kBranchDamping) {
+            // the Z member keeps (1 - gamma) c_Z; the I member receives gamma c_Z
+            const bool zi = zbit<W>(k, g.a) != 0u;
+            const double cz = zi ? ci : cj, cI = zi ? cj : ci;
+            const double nz = __dmul_rn(cz, g.cs), nI = __dadd_rn(cI, __dmul_rn(cz, g.sn));
+            ni = zi ? nz : nI;
+            nj = zi ? nI : nz;
+          } else {
+            const double ei = rot_eps<W>(k, g), ej = rot_eps<W>(kj, g);
+            ni = __dadd_rn(__dmul_rn(ci, g.cs), __dmul_rn(cj, ej * g.sn));
+            nj = __dadd_rn(__dmul_rn(cj, g.cs), __dmul_rn(ci, ei * g.sn));
+          }
           bool di = false, dj = false;
           if (g.trunc) {
             di = drop_term<W>(k, ni, t);
@@ -423,17 +437,20 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
       u32 idx = 0, fl = 1u;
       if (w < hi) load_rec<W>(wl, w, k, c, idx, fl);
       if (fl == 0u) {
+        // unpaired: the parent is scaled in place and its child survives
+        // unless truncated (damping: only Z members branch, I members stay)
+        const bool branches = g.mode != kBranchDamping || zbit<W>(k, g.a) != 0u;
         const double eps = rot_eps<W>(k, g);
-        const double cp = __dmul_rn(c, g.cs);
+        const double cp = branches ? __dmul_rn(c, g.cs) : c;
         const double cc = __dmul_rn(c, eps * g.sn);
         bool dp = false;
-        keep = true;
+        keep = branches;
         if (g.trunc) {
           dp = drop_term<W>(k, cp, t);
           rot_flip<W>(k, g);
-          keep = !drop_term<W>(k, cc, t);
+          keep = branches && !drop_term<W>(k, cc, t);
         }
-        coef[idx] = dp ? dead_value() : cp;
+        if (dp || branches) coef[idx] = dp ? dead_value() : cp;
         kills += (u64)dp;
       }
       const u64 m = __ballot(keep);

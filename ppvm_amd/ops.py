@@ -23,6 +23,7 @@ X, Y, Z, H, S, S_DAG, SQRT_X, SQRT_X_DAG, SQRT_Y, SQRT_Y_DAG = range(1, 11)
 CNOT, CZ, CY = 11, 12, 13
 RX, RY, RZ, RXX, RYY, RZZ = 20, 21, 22, 23, 24, 25
 PAULI_ERROR, DEPOLARIZE1, DEPOLARIZE2, PAULI_ERROR2 = 40, 41, 42, 43
+AMPLITUDE_DAMPING = 44
 TRUNCATE = 60
 
 CLIFFORD_1Q = {

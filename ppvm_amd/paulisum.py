@@ -293,6 +293,14 @@ class PauliSum:
         if tr:
             self._h.truncate()
 
+    def amplitude_damping(self, addr0: int, gamma: float, *, truncate: bool = True):
+        """Amplitude damping (ppvm-python/src/ppvm/paulisum.py:389-403): X and Y on
+        the qubit are damped by sqrt(1 - gamma); Z branches into
+        gamma * I + (1 - gamma) * Z."""
+        _native.check(_native.lib().ppvm_amplitude_damping(self._h.h, int(addr0), float(gamma)))
+        if truncate:
+            self._h.truncate()
+
     def depolarize2(self, *args, p=None, truncate: bool = True):
         t, pv, tr = _split(args, p, "p", truncate)
         if len(t) % 2:

@@ -229,3 +229,42 @@ def test_trace_patterns(n):
     for bad in ("A", "[]", "X*Y"):
         with pytest.raises(ValueError):
             g.trace(bad)
+
+
+@pytest.mark.parametrize("n,tau", [(5, 0.0), (5, 1e-3), (70, 1e-4), (130, 1e-4)])
+def test_amplitude_damping(n, tau):
+    """Amplitude damping mixed with rotations, vs the oracle after every op:
+    identical keys, coefficient bits and order.  The start set holds I and Z
+    versions of the same words so that damped Z terms merge into existing I terms."""
+    rng = np.random.default_rng(500 + n)
+    base = random_terms(rng, n, 40)
+    strings = list(base)
+    for w in base[:20]:
+        q = int(rng.integers(0, n))
+        strings.append(w[:q] + "Z" + w[q + 1:])
+        strings.append(w[:q] + "I" + w[q + 1:])
+    strings = list(dict.fromkeys(strings))
+    g, r = _pair(n, strings, rng.normal(size=len(strings)).tolist(), tau=tau)
+    p = O.Program()
+    for i in range(24):
+        q = int(rng.integers(0, n))
+        if i % 3 == 0:
+            p.add(O.AMPLITUDE_DAMPING, q, p=[float(rng.uniform(0.05, 0.6))])
+        else:
+            p.add([O.RX, O.RZ, O.H][i % 3], q, theta=float(rng.uniform(-2, 2)))
+        if tau > 0:
+            p.truncate()
+    prog = p.array()
+    marks = list(range(len(prog)))
+    run_segments(g, r, prog, marks, ctx=f"damping n={n}")
+
+
+def test_amplitude_damping_python_api():
+    from ppvm_amd import PauliSum
+    s = PauliSum(["ZI"], 2, coefficients=[1.0])
+    s.amplitude_damping(0, 0.2)
+    t = dict(s.terms)
+    assert t["ZI"] == pytest.approx(0.8) and t["II"] == pytest.approx(0.2)
+    s = PauliSum(["ZI"], 2, coefficients=[1.0])
+    s.amplitude_damping(0, 0.0)
+    assert dict(s.terms) == {"ZI": 1.0}

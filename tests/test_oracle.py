@@ -446,3 +446,52 @@ def test_trace_closed_forms():
         if set(w) <= {"I", "X"}:
             want += c
     assert s.trace("X?*") == want
+
+
+# ---- amplitude damping -----------------------------------------------------------------------
+def _damp(s, q, gamma):
+    s.apply(O.Program().add(O.AMPLITUDE_DAMPING, q, p=[gamma]).array())
+
+
+def test_amplitude_damping_reference_vectors():
+    """ppvm-python/test/test_basics.py:390-414 (restated; the Python class
+    truncates after the channel with its default min_abs_coeff 1e-10)."""
+    def run(word, q, g):
+        s = _sum(2, [(word, 1.0)], tau=1e-10)
+        s.apply(O.Program().add(O.AMPLITUDE_DAMPING, q, p=[g]).truncate().array())
+        return dict(s.terms())
+    assert run("ZI", 0, 0.0) == {"ZI": 1.0}
+    g = 0.2
+    t = run("ZI", 0, g)
+    assert t["ZI"] == pytest.approx(1.0 - g) and t["II"] == pytest.approx(g)
+    assert run("XI", 0, g) == {"XI": pytest.approx((1.0 - g) ** 0.5)}
+    assert run("ZI", 1, g) == {"ZI": 1.0}
+
+
+def test_amplitude_damping_matches_kraus_adjoint():
+    """E^dag(P) = sum_i K_i^dag P K_i with K0 = diag(1, sqrt(1-g)), K1 = sqrt(g)|0><1|,
+    on random 3-qubit sums (I and Z versions of a word present, so children merge)."""
+    rng = np.random.default_rng(33)
+    n = 3
+    for trial in range(6):
+        g = float(rng.uniform(0.05, 0.95))
+        q = int(rng.integers(0, n))
+        strs = list({"".join("IXYZ"[v] for v in rng.integers(0, 4, n)) for _ in range(14)})
+        coefs = rng.normal(size=len(strs))
+        s = _sum(n, list(zip(strs, coefs.tolist())))
+        _damp(s, q, g)
+        K0 = np.array([[1, 0], [0, math.sqrt(1 - g)]], dtype=complex)
+        K1 = np.array([[0, math.sqrt(g)], [0, 0]], dtype=complex)
+
+        def on_q(K):
+            m = np.array([[1.0 + 0j]])
+            for i in range(n):
+                m = np.kron(m, K if i == q else np.eye(2))
+            return m
+        A = sum(c * pmat(w) for w, c in zip(strs, coefs))
+        E = on_q(K0).conj().T @ A @ on_q(K0) + on_q(K1).conj().T @ A @ on_q(K1)
+        want = decompose(E, n)
+        got = dict(s.terms())
+        assert set(k for k, v in got.items() if abs(v) > 1e-12) == set(want)
+        for k, v in want.items():
+            assert abs(got[k] - v) < 1e-12
