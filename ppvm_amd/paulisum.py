@@ -81,6 +81,15 @@ def _split(args, value, name, truncate):
     return [int(t) for t in args], value, truncate
 
 
+def _rust_fixed3(v: float) -> str:
+    """Rust's `{:.3}` for f64."""
+    if v != v:
+        return "NaN"
+    if v in (float("inf"), float("-inf")):
+        return "inf" if v > 0 else "-inf"
+    return f"{v:.3f}"
+
+
 class PauliSum:
     """A weighted sum of Pauli strings propagated on the GPU.
 
@@ -177,11 +186,12 @@ class PauliSum:
         return a == b
 
     def __str__(self) -> str:
-        # "{:.3} * {word}" lines sorted by weight (crates/ppvm-pauli-sum/src/sum/display.rs:40)
+        # "{:.3} * {word}" joined by " + ", stably sorted by weight
+        # (crates/ppvm-pauli-sum/src/sum/display.rs:29-44); Rust's {:.3} is fixed-point
         w = self.weights()
         t = dict(self.terms)
         rows = sorted(w, key=lambda sw: sw[1])
-        return "\n".join(f"{t[s]:.3} * {s}" for s, _ in rows)
+        return " + ".join(f"{_rust_fixed3(t[s])} * {s}" for s, _ in rows)
 
     # ---- observables --------------------------------------------------------------
     def overlap_with_zero(self) -> float:
