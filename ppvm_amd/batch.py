@@ -280,7 +280,7 @@ def run_hybrid(batch: Batch, threads: int = 256, use_graph: bool = True, engine_
 
 
 def autotune(workloads, caps=(1 << 16, 1 << 18), threads=(256, 1024), graphs=(True,),
-             engine_threads=(4, 8), cache_path: str | None = None, log=None) -> dict:
+             engine_threads=(4, 8, 16), cache_path: str | None = None, log=None) -> dict:
     """Measure candidate settings on the batch itself and return the fastest.
 
     For each arena cap the first trial runs everything through the persistent
