@@ -449,9 +449,77 @@ def run_surrogate(args) -> dict:
     }
 
 
+def _term_fingerprint(x, z, c) -> dict:
+    """Order-independent fingerprint of a term set: count and two wrap-around
+    u64 sums of a 64-bit mix of each key (alone, and with its coefficient bits)."""
+    def mix(v):
+        v = (v ^ (v >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        v = (v ^ (v >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return v ^ (v >> np.uint64(31))
+    with np.errstate(over="ignore"):
+        h = np.full(len(c), np.uint64(0x9E3779B97F4A7C15))
+        for col in list(np.asarray(x, dtype=np.uint64).T) + list(np.asarray(z, dtype=np.uint64).T):
+            h = mix(h ^ col)
+        hc = mix(h ^ np.ascontiguousarray(c, dtype=np.float64).view(np.uint64))
+        return {"terms": int(len(c)), "key_sum": int(h.sum(dtype=np.uint64)),
+                "key_coef_sum": int(hc.sum(dtype=np.uint64))}
+
+
+def run_parity(args) -> dict:
+    """Full-size parity of the headline workload: the whole 20-step 10x10
+    lattice propagation (52M live terms) on the GPU against the oracle's
+    sharded multi-thread variant on the host cores (same term set and
+    coefficient bits; the sharded variant stores terms in another order, so
+    the comparison is by order-independent fingerprints), plus the full-size
+    CPU rate."""
+    from oracle.pyoracle import OracleSum
+    from ppvm_amd import workloads as WL
+    w = WL.lattice_tfim()
+    t0 = time.perf_counter()
+    g = _make(w)
+    g.apply(w.program)
+    g.synchronize()
+    gpu_s = time.perf_counter() - t0
+    gx, gz, gc = g._export()
+    g_ovl = g.overlap_with_zero()
+    g_u = g._h.counters()["term_updates"]
+    del g
+    fg = _term_fingerprint(gx, gz, gc)
+    del gx, gz, gc
+    _log(f"gpu: {gpu_s:.2f}s, {fg['terms']} terms")
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = OracleSum(w.n_qubits, w.min_abs_coeff, w.max_pauli_weight)
+    o.add_strings([s for s, _ in w.observable], [c for _, c in w.observable])
+    t0 = time.perf_counter()
+    prev = 0
+    for li, m in enumerate(w.layer_marks):  # one call per layer, so progress shows
+        o.apply_sharded(w.program[prev:m + 1], threads)
+        prev = m + 1
+        if li % 3 == 2:
+            _log(f"cpu sharded: step {li // 3 + 1} done, {len(o)} terms, {time.perf_counter() - t0:.1f}s")
+    cpu_s = time.perf_counter() - t0
+    ox, oz, oc = o.export()
+    fo = _term_fingerprint(ox, oz, oc)
+    o_ovl = float(np.sum(oc[~np.any(ox != 0, axis=1)]))
+    _log(f"cpu sharded x{threads}: {cpu_s:.1f}s, {fo['terms']} terms")
+    return {
+        "metric": "full-size parity: 10x10 lattice TFIM, GPU vs sharded CPU oracle",
+        "value": 1.0 if fg == fo else 0.0,
+        "unit": "identical term set and coefficient bits (1 = yes)",
+        "n_gpus": 1,
+        "config": {"workload": "lattice_tfim: 10x10 open TFIM, J=1, h=3.04, dt=0.05, 20 Trotter steps, "
+                               "Z(5,5), threshold 1e-5, max weight 10, truncate after every gate"},
+        "gpu": {"seconds": gpu_s, "fingerprint": fg, "overlap_with_zero": g_ovl, "term_updates": g_u},
+        "cpu_sharded": {"seconds": cpu_s, "threads": threads, "fingerprint": fo,
+                        "overlap_with_zero": o_ovl, "term_updates": o.term_updates(),
+                        "rate": o.term_updates() / cpu_s, "cpu_model": cpu_model()},
+        "overlap_abs_diff": abs(g_ovl - o_ovl),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--mode", choices=["lattice", "batched", "surrogate"], default="lattice")
+    ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity"], default="lattice")
     ap.add_argument("--verify", type=int, default=16, help="batched: instances re-run on the engine")
     ap.add_argument("--cpu-instances", type=int, default=256, help="batched: at most this many instances on the CPU")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="batched: CPU baseline time budget (s)")
@@ -465,7 +533,8 @@ def main() -> int:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of "
                                    "this build (--gpus must be 1)", "n_gpus": args.gpus}))
         return 2
-    res = {"lattice": run, "batched": run_batched, "surrogate": run_surrogate}[args.mode](args)
+    res = {"lattice": run, "batched": run_batched, "surrogate": run_surrogate,
+           "parity": run_parity}[args.mode](args)
     print(json.dumps(res), flush=True)
     return 0
 
