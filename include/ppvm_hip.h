@@ -210,7 +210,8 @@ int ppvm_get_stats(ppvm_handle h, ppvm_stats* out);
  * pairs, appended children, truncated terms, map-program entries, map
  * launches, bit-sliced planes read by scans (entries x planes), capacity,
  * syncs, regrows, compactions, peak live terms, entries transposed into the
- * bit-sliced planes, reserved (0). */
+ * bit-sliced planes, worklist records that the partitioned pairing sent to
+ * the global table (buckets too large for one workgroup's LDS). */
 int ppvm_get_counters(ppvm_handle h, uint64_t out[18]);
 
 /* Per-phase kernel timing (HIP events on the handle's stream), for the

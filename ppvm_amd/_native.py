@@ -225,7 +225,7 @@ class Handle:
 
     COUNTERS = ("term_updates", "gates", "rotations", "scanned", "wl_total", "pairs_total",
                 "children_total", "kills_total", "map_entries", "map_launches", "scan_planes",
-                "capacity", "syncs", "regrows", "compactions", "peak_live", "transposed", "reserved")
+                "capacity", "syncs", "regrows", "compactions", "peak_live", "transposed", "pair_fallback")
 
     def counters(self) -> dict:
         out = np.zeros(18, dtype=np.uint64)

@@ -12,7 +12,7 @@ struct alignas(16) WLRec {
   u64 k[2 * W];
   double c;
   u32 idx;
-  u32 flags;  // 1 = merged with its partner by k_rot_pair
+  u32 flags;  // 1 = merged with its partner by k_pair_lds, 2 = dead entry
 };
 
 // Worklist record load with 16-byte vector loads.

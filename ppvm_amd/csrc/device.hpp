@@ -67,7 +67,8 @@ struct alignas(64) DevState {
   double overlap;     // last overlap_with_zero result
   u64 tvw;            // words of the bit-sliced planes that are current (entries < 64 tvw)
   u64 transposed;     // entries written into the bit-sliced planes
-  u64 pad[5];
+  u64 pair_fallback;  // records the partitioned pairing sent to the global table
+  u64 pad[4];
 };
 
 // Branching op: a rotation exp(-i theta/2 G) with G acting on qubits a (and

@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <unordered_map>
@@ -119,6 +120,9 @@ struct ppvm_sum {
   u64 tcap = 0;
   u64* masks = nullptr;     // ballot masks, cap/64 + kMaxParts words
   u64* tp = nullptr;        // bit-sliced key planes, 2 n planes of tstride words (k_rot_count)
+  u64* hsh = nullptr;       // pair-key hash per worklist record (k_rot_gather)
+  u64* bkt = nullptr;       // worklist records in bucket order (k_pair_scatter)
+  u32* pmeta = nullptr;     // per-(bucket, slice) counts, then bucket bases and sizes
   u64 tstride = 0;
   u64* parts = nullptr;     // 6 x kMaxParts per-chunk counts / partial sums
   double* partials = nullptr;
@@ -135,6 +139,9 @@ struct ppvm_sum {
   std::vector<Issued> log;
   int issued_since_sync = 0;
   u64 n_bound = 0;  // host-side upper bound of arena entries (grid sizing)
+  u64 n_synced = 0; // arena entries at the last host synchronisation (path choice)
+  u64 partition_min = 0;  // pair through the radix partition from this many entries on
+  u32 bucket_lg_max = 0;  // at most 2^bucket_lg_max buckets in the partition
   ppvm_stats hs{};
   // profiling
   bool prof = false;
@@ -217,10 +224,14 @@ void free_buffers(ppvm_sum* s) {
   if (s->table) (void)hipFree(s->table);
   if (s->masks) (void)hipFree(s->masks);
   if (s->tp) (void)hipFree(s->tp);
+  if (s->hsh) (void)hipFree(s->hsh);
+  if (s->bkt) (void)hipFree(s->bkt);
   s->wl = nullptr;
   s->table = nullptr;
   s->masks = nullptr;
   s->tp = nullptr;
+  s->hsh = nullptr;
+  s->bkt = nullptr;
 }
 
 // The bit-sliced planes no longer match the arena (new buffers, rewritten or
@@ -269,16 +280,21 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
   if (s->table) (void)hipFree(s->table);
   if (s->masks) (void)hipFree(s->masks);
   if (s->tp) (void)hipFree(s->tp);
+  if (s->hsh) (void)hipFree(s->hsh);
+  if (s->bkt) (void)hipFree(s->bkt);
   s->wl = nullptr;
   s->table = nullptr;
   s->masks = nullptr;
   s->tp = nullptr;
+  s->hsh = nullptr;
+  s->bkt = nullptr;
   s->tcap = pow2_at_least(2 * new_cap);
   s->tstride = new_cap / 64 + 8;  // multiple of 8 words; room for the last 8-word group
   if (hipMalloc(&s->wl, new_cap * s->wl_rec()) != hipSuccess ||
       hipMalloc(&s->table, s->tcap * 8) != hipSuccess ||
       hipMalloc(&s->masks, (new_cap / 64 + kMaxParts + 64) * 8) != hipSuccess ||
-      hipMalloc(&s->tp, 2 * (size_t)s->n * s->tstride * 8) != hipSuccess) {
+      hipMalloc(&s->tp, 2 * (size_t)s->n * s->tstride * 8) != hipSuccess ||
+      hipMalloc(&s->hsh, new_cap * 8) != hipSuccess || hipMalloc(&s->bkt, new_cap * 8) != hipSuccess) {
     (void)hipGetLastError();
     return fail(PPVM_ENOMEM, "device allocation of work buffers for %llu terms failed",
                 (unsigned long long)new_cap);
@@ -310,6 +326,28 @@ u32 chunk_blocks(u64 bound, u32 target) {
 constexpr int kFixedGrid = 2048;  // grid-stride kernels with per-block partials
 constexpr u32 kArenaGrid = 2048;  // blocks of the chunked arena passes (count, gather)
 constexpr u32 kListGrid = 1024;   // blocks of the chunked worklist passes (emit)
+constexpr u32 kPairGrid = 512;    // blocks of k_pair_lds (2 resident per CU; each loops over buckets)
+// Arena size (at the last synchronisation) from which rotations pair through
+// the radix partition instead of the single global-table kernel; the
+// environment variable PPVM_PARTITION_MIN_ENTRIES overrides it per handle
+// (both paths give identical results; 0 forces the partition everywhere).
+constexpr u64 kPartitionEntries = 4u << 20;
+u64 partition_min_entries() {
+  const char* e = std::getenv("PPVM_PARTITION_MIN_ENTRIES");
+  if (!e || !*e) return kPartitionEntries;
+  char* end = nullptr;
+  const unsigned long long v = std::strtoull(e, &end, 10);
+  return (end && *end == '\0') ? (u64)v : kPartitionEntries;
+}
+// Largest bucket count (log2) of the partition; PPVM_PARTITION_MAX_LG lowers it
+// (fewer, larger buckets take the multi-pass and global-table branches).
+u32 bucket_lg_limit() {
+  const char* e = std::getenv("PPVM_PARTITION_MAX_LG");
+  if (!e || !*e) return kMaxBucketsLg;
+  char* end = nullptr;
+  const unsigned long v = std::strtoul(e, &end, 10);
+  return (end && *end == '\0' && v < kMaxBucketsLg) ? (u32)v : kMaxBucketsLg;
+}
 
 int launch_finish(ppvm_sum* s, const FinishArgs& a) {
   int rc;
@@ -335,13 +373,32 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_GATHER))) return rc;
   hipLaunchKernelGGL(k_rot_gather<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, coef, s->cap, s->st, in,
-                     s->masks, s->part(0), wl);
+                     s->masks, s->part(0), wl, it.rot, s->hsh);
   if ((rc = prof_end(s))) return rc;
   const u32 tg = next_tab_gen(s);
-  const u32 Gp = (u32)grid_for(s->n_bound, kBlock * 2, kFixedGrid);
+  u32 Gp;
   if ((rc = prof_begin(s, PH_PAIR))) return rc;
-  hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
-                     it.rot, s->table, s->tcap, tg, s->trunc_spec(), it.seq + 1, s->part(1), s->part(2));
+  if (s->n_synced >= s->partition_min) {
+    // large gate: radix partition + LDS pairing
+    Gp = kPairGrid;
+    u32* hist = s->pmeta;
+    u32* bmeta = s->pmeta + (size_t)kMaxBuckets * kPairSlices;
+    const u32 lgm = s->bucket_lg_max;
+    hipLaunchKernelGGL(k_pair_hist, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st, in,
+                       s->cap, lgm, it.seq + 1, hist);
+    hipLaunchKernelGGL(k_pair_colscan, dim3((1u << lgm) / 64 + 1), dim3(kPartThreads), 0, s->stream, s->st, in,
+                       s->cap, lgm, hist, bmeta);
+    hipLaunchKernelGGL(k_pair_scatter, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st, in,
+                       s->cap, lgm, hist, s->bkt, bmeta);
+    hipLaunchKernelGGL(k_pair_lds<W>, dim3(Gp), dim3(kPairThreads), 0, s->stream, coef, s->cap, wl, s->st, in,
+                       it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(), s->part(1),
+                       s->part(2));
+  } else {
+    Gp = (u32)grid_for(s->n_bound, kBlock * 2, kFixedGrid);
+    hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
+                       it.rot, s->hsh, s->table, s->tcap, tg, s->trunc_spec(), it.seq + 1, s->part(1),
+                       s->part(2));
+  }
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_ECOUNT))) return rc;
   hipLaunchKernelGGL(k_emit_count<W>, dim3(Gl), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
@@ -596,6 +653,7 @@ int sync_check(ppvm_sum* s) {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, &s->st->overflow, 1ull, 0ull);
     HIPCHK(hipGetLastError());
     s->n_bound = c.n_entries;
+    s->n_synced = s->n_bound;
     std::vector<Issued> redo;
     for (auto& it : s->log)
       if (it.seq >= seq) redo.push_back(it);
@@ -610,6 +668,7 @@ int sync_check(ppvm_sum* s) {
   s->issued_since_sync = 0;
   const CountSlot& c = s->st_host->slot[s->op_seq & 1];
   s->n_bound = c.n_entries;
+  s->n_synced = s->n_bound;
   const u64 live = c.n_entries - c.n_dead;
   s->hs.peak_live = std::max(s->hs.peak_live, live);
   // compaction when dead entries dominate the scans
@@ -624,6 +683,7 @@ int sync_check(ppvm_sum* s) {
       return rc;
     }
     s->n_bound = s->st_host->slot[s->op_seq & 1].n_entries;
+    s->n_synced = s->n_bound;
   }
   // proactive growth: keep room for a full gate of children
   const CountSlot& c2 = s->st_host->slot[s->op_seq & 1];
@@ -951,6 +1011,7 @@ int upload(ppvm_sum* s, const std::vector<u64>& keys, const std::vector<double>&
   if ((rc = invalidate_tp(s))) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
   s->n_bound = n;
+  s->n_synced = s->n_bound;
   return PPVM_OK;
 }
 
@@ -983,6 +1044,8 @@ int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint,
   s->tau = tau;
   s->maxw = maxw;
   s->trunc_active = tau > 0.0 || maxw != ~0ull;
+  s->partition_min = partition_min_entries();
+  s->bucket_lg_max = bucket_lg_limit();
   s->clean = true;
   HIPCHK(hipGetDevice(&s->device));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -992,7 +1055,8 @@ int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint,
   if (hipMalloc(&s->st, sizeof(DevState)) != hipSuccess ||
       hipHostMalloc(&s->st_host, sizeof(DevState), hipHostMallocDefault) != hipSuccess ||
       hipMalloc(&s->partials, kOverlapGrid * sizeof(double)) != hipSuccess ||
-      hipMalloc(&s->parts, 6 * (size_t)kMaxParts * sizeof(u64)) != hipSuccess) {
+      hipMalloc(&s->parts, 6 * (size_t)kMaxParts * sizeof(u64)) != hipSuccess ||
+      hipMalloc(&s->pmeta, ((size_t)kMaxBuckets * kPairSlices + 2 * kMaxBuckets) * sizeof(u32)) != hipSuccess) {
     (void)hipGetLastError();
     delete s;
     return fail(PPVM_ENOMEM, "device allocation of handle state failed");
@@ -1045,6 +1109,7 @@ int ppvm_destroy(ppvm_handle h) {
   if (h->st_host) (void)hipHostFree(h->st_host);
   if (h->partials) (void)hipFree(h->partials);
   if (h->parts) (void)hipFree(h->parts);
+  if (h->pmeta) (void)hipFree(h->pmeta);
   if (h->preserve) (void)hipFree(h->preserve);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1105,6 +1170,9 @@ int ppvm_clone(ppvm_handle h, ppvm_handle* out) {
   HIPCHK(hipStreamSynchronize(h->stream));
   HIPCHK(hipMemcpy(&c->st->slot[c->op_seq & 1], &cs, sizeof(CountSlot), hipMemcpyHostToDevice));
   c->n_bound = cs.n_entries;
+  c->n_synced = c->n_bound;
+  c->partition_min = h->partition_min;
+  c->bucket_lg_max = h->bucket_lg_max;
   c->clean = h->clean;
   c->preserve_host = h->preserve_host;
   if ((rc = upload_preserve(c))) {
@@ -1330,7 +1398,7 @@ int ppvm_get_counters(ppvm_handle h, uint64_t out[18]) {
                      d.wl_total,     d.pairs_total,    d.children_total, d.kills_total,
                      d.map_entries,  d.map_launches,   d.scan_planes, h->cap,
                      h->hs.syncs,    h->hs.regrows,    h->hs.compactions, h->hs.peak_live,
-                     d.transposed,   0};
+                     d.transposed,   d.pair_fallback};
   for (int i = 0; i < 18; ++i) out[i] = v[i];
   return PPVM_OK;
 }

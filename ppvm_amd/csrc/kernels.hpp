@@ -6,10 +6,11 @@
 //   k_rot_gather : copy each anticommuting entry (full key, coefficient, arena
 //                  index) into the worklist, in arena order (dead entries are
 //                  flagged, not copied out).
-//   k_rot_pair   : each worklist record inserts its pair key (key with the
-//                  generator's pivot bit cleared) into a generation-tagged
-//                  open-addressed table; the second member of a pair to arrive
-//                  computes both merged coefficients
+//   k_pair_hist, k_pair_scatter, k_pair_lds : radix-partition the worklist
+//                  on the hash of the pair key (key with the generator's pivot
+//                  bit cleared) and pair each bucket in an LDS hash table; the
+//                  second member of a pair to arrive computes both merged
+//                  coefficients
 //                    c_k' = c_k cos + eps_p c_p sin,  c_p' = c_p cos + eps_k c_k sin
 //                  and writes them in place (with fused truncation).
 //   k_emit_count : unpaired records scale their parent in place and decide
@@ -266,17 +267,26 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ pl
   }
 }
 
+// Hash of a record's pair key as the pairing kernels use it; kDeadHash marks a
+// dead entry (a live key that hashes to it is moved one below).
+constexpr u64 kDeadHash = ~0ull;
+__device__ __forceinline__ u64 pair_hash(u64 h, bool dead) {
+  return dead ? kDeadHash : (h == kDeadHash ? kDeadHash - 1ull : h);
+}
+
 // ---- k_rot_gather ---------------------------------------------------------------
-// Copies every anticommuting entry into the worklist in arena order; records
-// of dead entries are kept (flags = 2) so the ranks match the masks.  Block 0
-// also publishes the worklist length.
+// Copies every anticommuting entry into the worklist in arena order, and the
+// hash of its pair key into hsh; records of dead entries are kept (flags = 2,
+// hash kDeadHash) so the ranks match the masks.  Block 0 also publishes the
+// worklist length.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ planes,
                                                        const double* __restrict__ coef, u64 cap,
                                                        DevState* st, u32 in_slot,
                                                        const u64* __restrict__ masks,
                                                        const u64* __restrict__ cnt,
-                                                       WLRec<W>* __restrict__ wl) {
+                                                       WLRec<W>* __restrict__ wl, RotSpec g,
+                                                       u64* __restrict__ hsh) {
   __shared__ u64 lds[kWaves];
   __shared__ u64 lm[kChunkMaxWords];
   __shared__ u32 loff[kChunkMaxWords];
@@ -309,18 +319,154 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ p
       u64 k[2 * W];
       load_key<W>(planes, cap, i, k);
       const double c = coef[i];
-      store_rec<W>(wl, base_off + r, k, c, (u32)i, is_dead(c) ? 2u : 0u);
+      const bool dead = is_dead(c);
+      store_rec<W>(wl, base_off + r, k, c, (u32)i, dead ? 2u : 0u);
+      u64 rep[2 * W];
+      pair_key<W>(k, g, rep);
+      hsh[base_off + r] = pair_hash(hash_key<W>(rep), dead);
     }
     __syncthreads();  // lm / loff are reused by the next chunk
   }
 }
 
+// ---- pairing ----------------------------------------------------------------------
+// Both members of a pair {P, P xor G} share the pair key and hence its 64-bit
+// hash h, which k_rot_gather writes per record (kDeadHash for dead entries).
+// Two implementations, chosen per gate by the host from the arena size:
+//  * k_rot_pair: every record inserts into one global open-addressed table
+//    (slot = gen(24) | fingerprint(8) | record(32), generation tags make
+//    clearing unnecessary) with device-scope CAS; cheapest for small gates.
+//  * the partitioned path: the worklist is radix-partitioned on the top lg
+//    bits of h into 2^lg buckets (k_pair_hist counts per (slice, bucket),
+//    k_pair_colscan turns the counts into per-slice offsets, k_pair_scatter
+//    places (low 32 bits of h, record) bucket-contiguously) and k_pair_lds
+//    pairs each bucket in an LDS table, so global memory sees a few 8-byte
+//    words per record plus the merged pairs' two records.
+// Bits of h on the partitioned path: bucket = top lg bits (lg <= 14); LDS
+// slot = bits 0..12; fingerprint = bits 13..24; sub-bin = bits 26..31.
+constexpr u32 kPairSlices = 256;     // worklist slices of the partition kernels
+constexpr u32 kPartThreads = 1024;   // threads per block of the partition kernels
+constexpr u32 kMaxBucketsLg = 14;
+constexpr u32 kMaxBuckets = 1u << kMaxBucketsLg;
+constexpr u32 kBucketTarget = 2048;  // records per bucket aimed at
+constexpr u32 kPairThreads = 512;    // threads per block of k_pair_lds
+constexpr u32 kLdsSlotsLg = 13;
+constexpr u32 kLdsSlots = 1u << kLdsSlotsLg;  // 32 KB of u32 slots
+constexpr u32 kPassMax = kLdsSlots / 2;       // records per LDS pass (load factor <= 1/2)
+constexpr u32 kPerThread = kPassMax / kPairThreads;
+constexpr u32 kSubBins = 64;
+constexpr u32 kPosLimit = (1u << 20) - 1;     // bucket positions held by a slot (20 bits)
+
+__device__ __forceinline__ u32 pair_bucket_lg(u64 n_wl, u32 lg_max) {
+  u32 lg = 0;
+  while (lg < lg_max && ((u64)kBucketTarget << lg) < n_wl) ++lg;
+  return lg;
+}
+__device__ __forceinline__ u32 bucket_of(u64 h, u32 lg) { return lg ? (u32)(h >> (64u - lg)) : 0u; }
+
+// Exclusive prefix of v over the threads of a kPartThreads block.
+__device__ __forceinline__ u32 block_excl_scan_u32(u32 v, u32* wsum) {
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  u32 incl = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const u32 y = __shfl_up(incl, o, 64);
+    if (lane >= (u32)o) incl += y;
+  }
+  if (lane == 63u) wsum[wave] = incl;
+  __syncthreads();
+  u32 pre = 0;
+  for (u32 w = 0; w < wave; ++w) pre += wsum[w];
+  __syncthreads();
+  return pre + incl - v;
+}
+
+// Records i, j whose fingerprints agree: if their pair keys agree too, both
+// merged coefficients are written in place
+//   c_i' = c_i cos + eps_j c_j sin,  c_j' = c_j cos + eps_i c_i sin
+// (damping: the Z member keeps c_Z cos, the I member receives c_I + c_Z sin),
+// with fused truncation, and both records are flagged merged.
+template <int W>
+__device__ __forceinline__ bool pair_try_merge(double* __restrict__ coef, WLRec<W>* __restrict__ wl, u64 i,
+                                               u64 j, const RotSpec& g, const TruncSpec& t, u64& kills) {
+  u64 k[2 * W], kj[2 * W];
+  double ci, cj;
+  u32 idxi, idxj, fli, flj;
+  load_rec<W>(wl, i, k, ci, idxi, fli);
+  load_rec<W>(wl, j, kj, cj, idxj, flj);
+  u64 rep[2 * W], repj[2 * W];
+  pair_key<W>(k, g, rep);
+  pair_key<W>(kj, g, repj);
+  bool same = true;
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p) same &= (repj[p] == rep[p]);
+  if (!same) return false;
+  double ni, nj;
+  if (g.mode == kBranchDamping) {
+    const bool zi = zbit<W>(k, g.a) != 0u;
+    const double cz = zi ? ci : cj, cI = zi ? cj : ci;
+    const double nz = __dmul_rn(cz, g.cs), nI = __dadd_rn(cI, __dmul_rn(cz, g.sn));
+    ni = zi ? nz : nI;
+    nj = zi ? nI : nz;
+  } else {
+    const double ei = rot_eps<W>(k, g), ej = rot_eps<W>(kj, g);
+    ni = __dadd_rn(__dmul_rn(ci, g.cs), __dmul_rn(cj, ej * g.sn));
+    nj = __dadd_rn(__dmul_rn(cj, g.cs), __dmul_rn(ci, ei * g.sn));
+  }
+  bool di = false, dj = false;
+  if (g.trunc) {
+    di = drop_term<W>(k, ni, t);
+    dj = drop_term<W>(kj, nj, t);
+  }
+  coef[idxi] = di ? dead_value() : ni;
+  coef[idxj] = dj ? dead_value() : nj;
+  wl[i].flags = 1u;
+  wl[j].flags = 1u;
+  kills += (u64)di + (u64)dj;
+  return true;
+}
+
+// Record i (pair-key hash h) into the global generation-tagged table: claims
+// an empty slot (first member of its pair so far) or merges with the member
+// already there.  Returns 1 if it merged.
+template <int W>
+__device__ __forceinline__ u64 pair_global(double* __restrict__ coef, WLRec<W>* __restrict__ wl, u64 i, u64 h,
+                                           u64* table, u64 tmask, u32 tgen, const RotSpec& g,
+                                           const TruncSpec& t, u64& kills) {
+  const u64 hg = mix64(h ^ 0x5bd1e9955bd1e995ull);
+  const u64 fp = hg >> 56;
+  const u64 mine = ((u64)tgen << 40) | (fp << 32) | i;
+  u64 pos = hg & tmask;
+  // a plain load may return a stale (older) value; that can only make a slot
+  // look empty, and the CAS then returns the value actually in memory
+  u64 sv = table[pos];
+  while (true) {
+    if ((u32)(sv >> 40) != tgen) {
+      const u64 old = cas_u64(&table[pos], sv, mine);
+      if (old == sv) return 0;
+      sv = old;
+      continue;
+    }
+    if (((sv >> 32) & 0xffull) == fp && pair_try_merge<W>(coef, wl, i, sv & 0xffffffffull, g, t, kills))
+      return 1;
+    pos = (pos + 1) & tmask;
+    sv = table[pos];
+  }
+}
+
+__device__ __forceinline__ u64 pair_table_mask(u64 n_wl, u64 tcap_max) {
+  u64 tcap = 64;
+  while (tcap < 2 * n_wl && tcap < tcap_max) tcap <<= 1;
+  return tcap - 1;
+}
+
 // ---- k_rot_pair -----------------------------------------------------------------
+// Global-table pairing, one record per thread (grid-stride).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, u64 cap,
                                                      WLRec<W>* __restrict__ wl, DevState* st,
-                                                     u32 in_slot, RotSpec g, u64* table,
-                                                     u64 tcap_max, u32 tgen, TruncSpec t,
+                                                     u32 in_slot, RotSpec g, const u64* __restrict__ hsh,
+                                                     u64* table, u64 tcap_max, u32 tgen, TruncSpec t,
                                                      u64 op_tag, u64* __restrict__ part_kills,
                                                      u64* __restrict__ part_pairs) {
   __shared__ u64 lds_sum[kWaves];
@@ -331,82 +477,349 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
     if (blockIdx.x == 0 && threadIdx.x == 0) cas_u64(&st->overflow, 0ull, op_tag);
     return;
   }
-  u64 tcap = 64;
-  while (tcap < 2 * n_wl && tcap < tcap_max) tcap <<= 1;
-  const u64 tmask = tcap - 1;
+  const u64 tmask = pair_table_mask(n_wl, tcap_max);
   u64 kills = 0, pairs = 0;
   const u64 stride = (u64)gridDim.x * kBlock;
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n_wl; i += stride) {
-    u64 k[2 * W];
-    double ci;
-    u32 idxi, fli;
-    load_rec<W>(wl, i, k, ci, idxi, fli);
-    if (fli == 2u) continue;  // dead entry
-    u64 rep[2 * W];
-    pair_key<W>(k, g, rep);
-    const u64 h = hash_key<W>(rep);
-    const u64 fp = h >> 56;
-    const u64 mine = ((u64)tgen << 40) | (fp << 32) | i;
-    u64 pos = h & tmask;
-    // Plain loads may return a stale (older) value; that can only make a slot
-    // look empty, and the CAS then returns the value actually in memory.
-    u64 s = table[pos];
-    while (true) {
-      if ((u32)(s >> 40) != tgen) {
-        const u64 old = cas_u64(&table[pos], s, mine);
-        if (old == s) break;  // claimed: first member of its pair (so far)
-        s = old;
-        continue;
-      }
-      if (((s >> 32) & 0xffull) == fp) {
-        const u64 j = s & 0xffffffffull;
-        u64 kj[2 * W];
-        double cj;
-        u32 idxj, flj;
-        load_rec<W>(wl, j, kj, cj, idxj, flj);
-        u64 repj[2 * W];
-        pair_key<W>(kj, g, repj);
-        bool same = true;
-#pragma unroll
-        for (int p = 0; p < 2 * W; ++p) same &= (repj[p] == rep[p]);
-        if (same) {
-          // second arrival: merge both members
-          double ni, nj;
-          if (g.mode == kBranchDamping) {
-            // the Z member keeps (1 - gamma) c_Z; the I member receives gamma c_Z
-            const bool zi = zbit<W>(k, g.a) != 0u;
-            const double cz = zi ? ci : cj, cI = zi ? cj : ci;
-            const double nz = __dmul_rn(cz, g.cs), nI = __dadd_rn(cI, __dmul_rn(cz, g.sn));
-            ni = zi ? nz : nI;
-            nj = zi ? nI : nz;
-          } else {
-            const double ei = rot_eps<W>(k, g), ej = rot_eps<W>(kj, g);
-            ni = __dadd_rn(__dmul_rn(ci, g.cs), __dmul_rn(cj, ej * g.sn));
-            nj = __dadd_rn(__dmul_rn(cj, g.cs), __dmul_rn(ci, ei * g.sn));
-          }
-          bool di = false, dj = false;
-          if (g.trunc) {
-            di = drop_term<W>(k, ni, t);
-            dj = drop_term<W>(kj, nj, t);
-          }
-          coef[idxi] = di ? dead_value() : ni;
-          coef[idxj] = dj ? dead_value() : nj;
-          wl[i].flags = 1u;
-          wl[j].flags = 1u;
-          kills += (u64)di + (u64)dj;
-          pairs += 1;
-          break;
-        }
-      }
-      pos = (pos + 1) & tmask;
-      s = table[pos];
-    }
+    const u64 h = hsh[i];
+    if (h != kDeadHash) pairs += pair_global<W>(coef, wl, i, h, table, tmask, tgen, g, t, kills);
   }
   const u64 bk = block_sum_u64(kills, lds_sum);
   const u64 bp = block_sum_u64(pairs, lds_sum);
   if (threadIdx.x == 0) {
     part_kills[blockIdx.x] = bk;
     part_pairs[blockIdx.x] = bp;
+  }
+}
+
+// ---- k_pair_hist ----------------------------------------------------------------
+// Slice s of the worklist counts its records per bucket: hist[s * B + b].
+// Also raises the overflow flag when the gate's children might not fit.
+__global__ __launch_bounds__(kPartThreads) void k_pair_hist(const u64* __restrict__ hsh, DevState* st,
+                                                           u32 in_slot, u64 cap, u32 lg_max, u64 op_tag,
+                                                           u32* __restrict__ hist) {
+  __shared__ u32 h[kMaxBuckets];
+  if (st->overflow) return;
+  const u64 n_entries = st->slot[in_slot].n_entries;
+  const u64 n_wl = st->n_wl;
+  if (n_entries + n_wl > cap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) cas_u64(&st->overflow, 0ull, op_tag);
+    return;
+  }
+  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  for (u32 b = threadIdx.x; b < nb; b += kPartThreads) h[b] = 0u;
+  __syncthreads();
+  const u64 lo = n_wl * blockIdx.x / kPairSlices, hi = n_wl * (blockIdx.x + 1) / kPairSlices;
+#pragma unroll 4
+  for (u64 r = lo + threadIdx.x; r < hi; r += kPartThreads) {
+    const u64 v = hsh[r];
+    if (v != kDeadHash) atomicAdd(&h[bucket_of(v, lg)], 1u);
+  }
+  __syncthreads();
+  u32* row = hist + (u64)blockIdx.x * nb;
+  for (u32 b = threadIdx.x; b < nb; b += kPartThreads) row[b] = h[b];
+}
+
+// ---- k_pair_colscan ---------------------------------------------------------------
+// Per bucket, the exclusive prefix of the slice counts (in place) and the
+// bucket size (bmeta[kMaxBuckets + b]).  A block owns 64 buckets; its 16 waves
+// each take 16 consecutive slices.
+constexpr u32 kScanGroups = kPartThreads / 64;
+constexpr u32 kSlicesPerGroup = kPairSlices / kScanGroups;
+__global__ __launch_bounds__(kPartThreads) void k_pair_colscan(const DevState* __restrict__ st, u32 in_slot,
+                                                              u64 cap, u32 lg_max, u32* __restrict__ hist,
+                                                              u32* __restrict__ bmeta) {
+  __shared__ u32 gsum[kScanGroups][64];
+  if (st->overflow) return;
+  const u64 n_entries = st->slot[in_slot].n_entries;
+  const u64 n_wl = st->n_wl;
+  if (n_entries + n_wl > cap) return;
+  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  const u32 b0 = blockIdx.x * 64u;
+  if (b0 >= nb) return;
+  const u32 lane = threadIdx.x & 63u, grp = threadIdx.x >> 6;
+  const u32 b = b0 + lane;
+  const bool on = b < nb;
+  const u32 s0 = grp * kSlicesPerGroup;
+  u32 v[kSlicesPerGroup], sum = 0;
+#pragma unroll
+  for (u32 q = 0; q < kSlicesPerGroup; ++q) {
+    v[q] = on ? hist[(u64)(s0 + q) * nb + b] : 0u;
+    sum += v[q];
+  }
+  gsum[grp][lane] = sum;
+  __syncthreads();
+  u32 run = 0;
+  for (u32 q = 0; q < grp; ++q) run += gsum[q][lane];
+  if (on) {
+#pragma unroll
+    for (u32 q = 0; q < kSlicesPerGroup; ++q) {
+      hist[(u64)(s0 + q) * nb + b] = run;
+      run += v[q];
+    }
+    if (grp == kScanGroups - 1u) bmeta[kMaxBuckets + b] = run;
+  }
+}
+
+// ---- k_pair_scatter -------------------------------------------------------------
+// Slice s writes bkt[pos] = record | (low 32 bits of h) << 32, at its offset
+// in the record's bucket: bucket base (prefix of the bucket sizes) plus the
+// counts of the slices before s.  Block 0 publishes the bucket bases in
+// bmeta[0..B).
+__global__ __launch_bounds__(kPartThreads) void k_pair_scatter(const u64* __restrict__ hsh,
+                                                              const DevState* __restrict__ st, u32 in_slot,
+                                                              u64 cap, u32 lg_max, const u32* __restrict__ hist,
+                                                              u64* __restrict__ bkt, u32* __restrict__ bmeta) {
+  __shared__ u32 off[kMaxBuckets];
+  __shared__ u32 wsum[kPartThreads / 64];
+  if (st->overflow) return;
+  const u64 n_entries = st->slot[in_slot].n_entries;
+  const u64 n_wl = st->n_wl;
+  if (n_entries + n_wl > cap) return;
+  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  const u32 s = blockIdx.x;
+  // bucket bases: exclusive scan of the sizes, kPer consecutive buckets per thread
+  constexpr u32 kPer = kMaxBuckets / kPartThreads;
+  const u32 b0 = threadIdx.x * kPer;
+  const u32* row = hist + (u64)s * nb;
+  u32 loc[kPer], pre[kPer], sum = 0;
+#pragma unroll
+  for (u32 q = 0; q < kPer; ++q) {
+    const bool on = b0 + q < nb;
+    loc[q] = on ? bmeta[kMaxBuckets + b0 + q] : 0u;
+    pre[q] = on ? row[b0 + q] : 0u;
+    sum += loc[q];
+  }
+  u32 base = block_excl_scan_u32(sum, wsum);
+#pragma unroll
+  for (u32 q = 0; q < kPer; ++q) {
+    if (b0 + q < nb) {
+      if (s == 0) bmeta[b0 + q] = base;
+      off[b0 + q] = base + pre[q];
+    }
+    base += loc[q];
+  }
+  __syncthreads();
+  const u64 lo = n_wl * s / kPairSlices, hi = n_wl * (s + 1) / kPairSlices;
+#pragma unroll 4
+  for (u64 r = lo + threadIdx.x; r < hi; r += kPartThreads) {
+    const u64 v = hsh[r];
+    if (v == kDeadHash) continue;
+    const u32 pos = atomicAdd(&off[bucket_of(v, lg)], 1u);
+    bkt[pos] = r | ((u64)(u32)v << 32);
+  }
+}
+
+// ---- k_pair_lds -----------------------------------------------------------------
+// One block per bucket (blocks loop over buckets).  A bucket of at most
+// kPassMax records is paired in one pass: (A) every thread loads its records'
+// bucket entries; (B) each record walks its linear probe sequence in the LDS
+// table and claims the first empty slot (slot = bucket position + 1 |
+// fingerprint << 20) or stops at the first slot with its fingerprint, which
+// makes it a candidate; (C) candidates are spread over the threads, load both
+// records and merge if the pair keys agree; (D) a candidate whose keys differ
+// (fingerprint collision) walks again from its home slot, testing every
+// fingerprint match, until it merges or claims a slot.  Claimed slots are
+// never released within a pass, so the probe sequence of a pair's second
+// member always passes the first member's slot.  Larger buckets are paired
+// in several passes over ranges of sub-bins; a single sub-bin that is still
+// too large, or a bucket whose positions exceed 20 bits, uses the global
+// table.  Pair members always share bucket and sub-bin.
+template <int W>
+__device__ __forceinline__ u64 lds_probe_sync(double* __restrict__ coef, WLRec<W>* __restrict__ wl,
+                                              u32* tab, const u64* __restrict__ bkt, u32 base, u32 e, u64 v,
+                                              const RotSpec& g, const TruncSpec& t, u64& kills) {
+  const u32 lo = (u32)(v >> 32);
+  const u32 i = (u32)v;
+  const u32 fp = (lo >> kLdsSlotsLg) & 0xfffu;
+  const u32 mine = (e + 1u) | (fp << 20);
+  u32 pos = lo & (kLdsSlots - 1u);
+  while (true) {
+    u32 sv = __hip_atomic_load(&tab[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (sv == 0u) {
+      u32 exp = 0u;
+      if (__hip_atomic_compare_exchange_strong(&tab[pos], &exp, mine, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP))
+        return 0;
+      sv = exp;
+    }
+    if ((sv >> 20) == fp && (sv & kPosLimit) != e + 1u) {
+      const u32 j = (u32)bkt[base + (sv & kPosLimit) - 1u];
+      if (pair_try_merge<W>(coef, wl, i, j, g, t, kills)) return 1;
+    }
+    pos = (pos + 1u) & (kLdsSlots - 1u);
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(kPairThreads) void k_pair_lds(double* __restrict__ coef, u64 cap,
+                                                           WLRec<W>* __restrict__ wl, DevState* st,
+                                                           u32 in_slot, RotSpec g, u32 lg_max,
+                                                           const u64* __restrict__ hsh,
+                                                           const u64* __restrict__ bkt,
+                                                           const u32* __restrict__ bmeta, u64* table,
+                                                           u64 tcap_max, u32 tgen, TruncSpec t,
+                                                           u64* __restrict__ part_kills,
+                                                           u64* __restrict__ part_pairs) {
+  __shared__ __align__(16) u32 tab[kLdsSlots];
+  __shared__ u32 cand[kPassMax];
+  __shared__ u32 sub[kSubBins];
+  __shared__ u32 plo[kSubBins], phi[kSubBins], pfb[kSubBins];
+  __shared__ u32 npass, ncand;
+  __shared__ u64 lds_sum[kPairThreads / 64];
+  if (st->overflow) return;
+  const u64 n_entries = st->slot[in_slot].n_entries;
+  const u64 n_wl = st->n_wl;
+  if (n_entries + n_wl > cap) return;
+  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  const u64 tmask = pair_table_mask(n_wl, tcap_max);
+  const u32 tid = threadIdx.x;
+  uint4* t4 = reinterpret_cast<uint4*>(tab);
+  u64 kills = 0, pairs = 0, fallback = 0;
+  for (u32 b = blockIdx.x; b < nb; b += gridDim.x) {
+    const u32 base = bmeta[b], T = bmeta[kMaxBuckets + b];
+    if (T == 0u) continue;
+    if (T <= kPassMax) {
+      // ---- single pass
+      for (u32 e = tid; e < kLdsSlots / 4; e += kPairThreads) t4[e] = make_uint4(0u, 0u, 0u, 0u);
+      if (tid == 0) ncand = 0u;
+      u64 v[kPerThread];
+#pragma unroll
+      for (u32 q = 0; q < kPerThread; ++q) {
+        const u32 e = tid + q * kPairThreads;
+        v[q] = e < T ? bkt[base + e] : 0ull;
+      }
+      __syncthreads();
+#pragma unroll
+      for (u32 q = 0; q < kPerThread; ++q) {
+        const u32 e = tid + q * kPairThreads;
+        if (e < T) {
+          const u32 lo = (u32)(v[q] >> 32);
+          const u32 fp = (lo >> kLdsSlotsLg) & 0xfffu;
+          const u32 mine = (e + 1u) | (fp << 20);
+          u32 pos = lo & (kLdsSlots - 1u);
+          while (true) {
+            u32 sv = __hip_atomic_load(&tab[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (sv == 0u) {
+              u32 exp = 0u;
+              if (__hip_atomic_compare_exchange_strong(&tab[pos], &exp, mine, __ATOMIC_RELAXED,
+                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP))
+                break;  // claimed
+              sv = exp;
+            }
+            if ((sv >> 20) == fp) {  // candidate partner
+              const u32 c = atomicAdd(&ncand, 1u);
+              cand[c] = e | (((sv & kPosLimit) - 1u) << 16);
+              break;
+            }
+            pos = (pos + 1u) & (kLdsSlots - 1u);
+          }
+        }
+      }
+      __syncthreads();
+      const u32 nc = ncand;
+      bool retry_any = false;
+      for (u32 c = tid; c < nc; c += kPairThreads) {
+        const u32 ce = cand[c] & 0xffffu, cj = cand[c] >> 16;
+        const u32 i = (u32)bkt[base + ce], j = (u32)bkt[base + cj];
+        if (pair_try_merge<W>(coef, wl, i, j, g, t, kills)) {
+          ++pairs;
+        } else {
+          cand[c] |= 0x80000000u;  // fingerprint collision: walk again below
+          retry_any = true;
+        }
+      }
+      if (__syncthreads_or(retry_any)) {
+        for (u32 c = tid; c < nc; c += kPairThreads) {
+          if (!(cand[c] & 0x80000000u)) continue;
+          const u32 ce = cand[c] & 0xffffu;
+          pairs += lds_probe_sync<W>(coef, wl, tab, bkt, base, ce, bkt[base + ce], g, t, kills);
+        }
+      }
+      __syncthreads();  // table and candidate list are reset for the next bucket
+      continue;
+    }
+    // ---- several passes (large bucket)
+    if (T > kPosLimit) {
+      if (tid == 0) {
+        plo[0] = 0u;
+        phi[0] = kSubBins;
+        pfb[0] = 1u;
+        npass = 1u;
+      }
+    } else {
+      if (tid < kSubBins) sub[tid] = 0u;
+      __syncthreads();
+      for (u32 e = tid; e < T; e += kPairThreads) atomicAdd(&sub[(u32)(bkt[base + e] >> 58)], 1u);
+      __syncthreads();
+      if (tid == 0) {
+        u32 np = 0, start = 0, acc = 0;
+        for (u32 q = 0; q < kSubBins; ++q) {
+          const u32 c = sub[q];
+          if (acc > 0u && acc + c > kPassMax) {
+            plo[np] = start;
+            phi[np] = q;
+            pfb[np] = acc > kPassMax ? 1u : 0u;
+            ++np;
+            start = q;
+            acc = 0u;
+          }
+          acc += c;
+        }
+        plo[np] = start;
+        phi[np] = kSubBins;
+        pfb[np] = acc > kPassMax ? 1u : 0u;
+        npass = np + 1u;
+      }
+    }
+    __syncthreads();
+    const u32 np = npass;
+    for (u32 q = 0; q < np; ++q) {
+      const u32 s0 = plo[q], s1 = phi[q];
+      if (!pfb[q]) {
+        for (u32 e = tid; e < kLdsSlots / 4; e += kPairThreads) t4[e] = make_uint4(0u, 0u, 0u, 0u);
+        __syncthreads();
+        for (u32 e = tid; e < T; e += kPairThreads) {
+          const u64 v = bkt[base + e];
+          const u32 sb = (u32)(v >> 58);
+          if (sb >= s0 && sb < s1) pairs += lds_probe_sync<W>(coef, wl, tab, bkt, base, e, v, g, t, kills);
+        }
+        __syncthreads();  // the table is cleared again by the next pass
+      } else {
+        for (u32 e = tid; e < T; e += kPairThreads) {
+          const u64 v = bkt[base + e];
+          const u32 sb = (u32)(v >> 58);
+          if (sb >= s0 && sb < s1) {
+            const u32 i = (u32)v;
+            pairs += pair_global<W>(coef, wl, i, hsh[i], table, tmask, tgen, g, t, kills);
+            ++fallback;
+          }
+        }
+      }
+    }
+    __syncthreads();  // plan arrays are rewritten for the next bucket
+  }
+  if (fallback) add_u64(&st->pair_fallback, fallback);
+  // deterministic block sums
+  for (int o = 32; o > 0; o >>= 1) {
+    kills += __shfl_down(kills, o, 64);
+    pairs += __shfl_down(pairs, o, 64);
+  }
+  __syncthreads();
+  if ((tid & 63u) == 0u) lds_sum[tid >> 6] = kills;
+  __syncthreads();
+  if (tid == 0) {
+    u64 s = 0;
+    for (u32 w = 0; w < kPairThreads / 64; ++w) s += lds_sum[w];
+    part_kills[blockIdx.x] = s;
+  }
+  __syncthreads();
+  if ((tid & 63u) == 0u) lds_sum[tid >> 6] = pairs;
+  __syncthreads();
+  if (tid == 0) {
+    u64 s = 0;
+    for (u32 w = 0; w < kPairThreads / 64; ++w) s += lds_sum[w];
+    part_pairs[blockIdx.x] = s;
   }
 }
 

@@ -17,6 +17,16 @@ from tests.helpers import (assert_same_terms, gpu_for, oracle_for, random_progra
 pytestmark = pytest.mark.gpu
 
 
+@pytest.fixture(autouse=True, params=["global", "partition"])
+def pairing_path(request, monkeypatch):
+    """Run every test with both pairing implementations: the single
+    global-table kernel and the radix partition + LDS kernels (forced at every
+    size).  Handles read the choice when they are created."""
+    monkeypatch.setenv("PPVM_PARTITION_MIN_ENTRIES",
+                       "0" if request.param == "partition" else str(2**63))
+    return request.param
+
+
 def _pair(n, strings, coeffs, tau=0.0, maxw=None, cap=0):
     from oracle.pyoracle import OracleSum
     from ppvm_amd import PauliSum
