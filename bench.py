@@ -46,8 +46,12 @@ def _make(w, cap: int = 0):
                         max_pauli_weight=w.max_pauli_weight, capacity_hint=cap)
 
 
-KERNELS = ["k_rot_count", "k_rot_gather", "k_rot_pair", "k_emit_count", "k_emit_write", "k_map",
+# Timed phases of the engine (HIP events per launch); "pairing" is k_rot_pair
+# on small gates and k_pair_hist + k_pair_colscan + k_pair_scatter + k_pair_lds
+# on large ones.
+KERNELS = ["k_rot_count", "k_rot_gather", "pairing", "k_emit_count", "k_emit_write", "k_map",
            "k_finish", "other"]
+PAIRING_KERNELS = ("k_rot_pair", "k_pair_hist", "k_pair_colscan", "k_pair_scatter", "k_pair_lds")
 
 
 def phase_bytes(cnt: dict, W: int, nq: int) -> dict:
@@ -59,8 +63,12 @@ def phase_bytes(cnt: dict, W: int, nq: int) -> dict:
     k_rot_count  : 1 bit per bit-sliced plane the generator needs for every
                    entry + N/8 B of ballot masks + N_t (16W + 2 nq / 8) for the
                    entries tested from the key planes and transposed.
-    k_rot_gather : N/8 B masks + N_a (S + R): key and coefficient read, record written.
-    k_rot_pair   : N_a R records read + 2 N_p 8 B coefficients written.
+    k_rot_gather : N/8 B masks + N_a (S + R + 8): key and coefficient read,
+                   record and pair-key hash written.
+    pairing      : N_a 8 B hashes read + N_p (2 R + 24): both records of each
+                   merged pair read, two coefficients and two flags written
+                   (the minimum of either implementation; the partition's own
+                   traffic shows in the measured bytes).
     k_emit_count : N_a R records read + (N_a - 2 N_p) 8 B coefficients written + N_a/8 masks.
     k_emit_write : N_a/8 masks + N_c (R + S): record read, child term written.
     """
@@ -72,8 +80,8 @@ def phase_bytes(cnt: dict, W: int, nq: int) -> dict:
     nch = cnt["children_total"]
     return {
         "k_rot_count": cnt["scan_planes"] / 8 + N / 8 + cnt["transposed"] * (16 * W + 2 * nq / 8),
-        "k_rot_gather": N / 8 + na * (S + R),
-        "k_rot_pair": na * R + 2 * npair * 8,
+        "k_rot_gather": N / 8 + na * (S + R + 8),
+        "pairing": na * 8 + npair * (2 * R + 24),
         "k_emit_count": na * R + (na - 2 * npair) * 8 + na / 8,
         "k_emit_write": na / 8 + nch * (R + S),
     }
@@ -95,16 +103,23 @@ def cpu_model() -> str:
 
 def measured_traffic(kernel: str):
     """HBM bytes per dispatch of `kernel` from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), or None."""
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel), or None.
+    For the "pairing" phase: the bytes of all its kernels per gate."""
+    rows = {}
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 5 and parts[0] == kernel:
-                    return float(parts[4])
-    except OSError:
-        pass
-    return None
+                if len(parts) == 5 and parts[0].startswith("k_"):
+                    rows[parts[0]] = (int(parts[1]), float(parts[4]))
+    except (OSError, ValueError):
+        return None
+    if kernel == "pairing":
+        gates = rows.get("k_rot_gather", (0, 0.0))[0]
+        if not gates:
+            return None
+        return sum(n * b for k, (n, b) in rows.items() if k in PAIRING_KERNELS) / gates
+    return rows[kernel][1] if kernel in rows else None
 
 
 def run(args) -> dict:
@@ -169,7 +184,9 @@ def run(args) -> dict:
             if nm in pb:
                 kern[nm]["alg_bytes_total"] = float(pb[nm])
                 kern[nm]["achieved_GBs"] = pb[nm] / (kern[nm]["ms_total"] * 1e-3) / 1e9
-    dom = max(kern, key=lambda k: kern[k]["ms_total"])
+    # the dominant kernel: the single kernel with the largest total time (the
+    # pairing phase is several kernels, each well below the gather)
+    dom = max((k for k in kern if k.startswith("k_") and k in pb), key=lambda k: kern[k]["ms_total"])
     _log("kernels: " + json.dumps(kern))
     dk = kern[dom]
     achieved = dk.get("achieved_GBs", 0.0)

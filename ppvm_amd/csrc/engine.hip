@@ -349,6 +349,13 @@ u32 bucket_lg_limit() {
   return (end && *end == '\0' && v < kMaxBucketsLg) ? (u32)v : kMaxBucketsLg;
 }
 
+// Arena size the launch grids of rotations and map programs are sized for.
+// Those kernels loop over chunks (or grid-stride), so a grid smaller than the
+// arena is only slower, never wrong; the host's hard bound doubles with every
+// rotation since the last synchronisation, so grids follow the synchronised
+// size with room for 4x growth instead.
+u64 grid_entries(const ppvm_sum* s) { return std::min<u64>(s->n_bound, 4 * s->n_synced + (1u << 16)); }
+
 int launch_finish(ppvm_sum* s, const FinishArgs& a) {
   int rc;
   if ((rc = prof_begin(s, PH_FINISH))) return rc;
@@ -362,8 +369,9 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
   int rc;
   // chunked kernels loop over their chunks, so the grids stay near one
   // resident wave of blocks however large the arena is
-  const u32 G = std::min<u32>(chunk_blocks(s->n_bound, kArenaChunk), kArenaGrid);
-  const u32 Gl = std::min<u32>(chunk_blocks(s->n_bound, kListChunk), kListGrid);
+  const u64 ne = grid_entries(s);
+  const u32 G = std::min<u32>(chunk_blocks(ne, kArenaChunk), kArenaGrid);
+  const u32 Gl = std::min<u32>(chunk_blocks(ne, kListChunk), kListGrid);
   WLRec<W>* wl = reinterpret_cast<WLRec<W>*>(s->wl);
   u64* planes = s->planes[s->cur];
   double* coef = s->coef[s->cur];
@@ -394,7 +402,7 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
                        it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(), s->part(1),
                        s->part(2));
   } else {
-    Gp = (u32)grid_for(s->n_bound, kBlock * 2, kFixedGrid);
+    Gp = (u32)grid_for(ne, kBlock * 2, kFixedGrid);
     hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
                        it.rot, s->hsh, s->table, s->tcap, tg, s->trunc_spec(), it.seq + 1, s->part(1),
                        s->part(2));
@@ -416,7 +424,7 @@ template <int W>
 int launch_map(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
   int rc;
-  const u32 Gm = (u32)grid_for(s->n_bound, kBlock * 4, kFixedGrid);
+  const u32 Gm = (u32)grid_for(grid_entries(s), kBlock * 4, kFixedGrid);
   if ((rc = prof_begin(s, PH_MAP))) return rc;
   hipLaunchKernelGGL(k_map<W>, dim3(Gm), dim3(kBlock), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
                      s->cap, s->st, in, it.map, s->trunc_spec(), s->part(0), s->part(1));

@@ -91,6 +91,38 @@ __device__ __forceinline__ u64 block_sum_prefix(const u64* __restrict__ v, u32 b
   return block_sum_all((s0 + s1) + (s2 + s3), lds);
 }
 
+// Sums of four arrays of lengths na, nb, nc, nd in one pass (loads of all four
+// in flight together); valid in thread 0.
+__device__ __forceinline__ void block_sums4(const u64* __restrict__ a, u32 na, const u64* __restrict__ b, u32 nb,
+                                            const u64* __restrict__ c, u32 nc, const u64* __restrict__ d, u32 nd,
+                                            u64* lds4, u64 (&out)[4]) {
+  u64 v[4] = {0, 0, 0, 0};
+  const u32 n = max(max(na, nb), max(nc, nd));
+  for (u32 j = threadIdx.x; j < n; j += kBlock) {
+    if (j < na) v[0] += a[j];
+    if (j < nb) v[1] += b[j];
+    if (j < nc) v[2] += c[j];
+    if (j < nd) v[3] += d[j];
+  }
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+  }
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) lds4[q * kWaves + wave] = v[q];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    u64 t = 0;
+    for (int w = 0; w < kWaves; ++w) t += lds4[q * kWaves + w];
+    out[q] = t;
+  }
+}
+
 // Sum of cnt[0..b) (the chunk's offset), returned to every thread.
 __device__ __forceinline__ u64 chunk_offset(const u64* __restrict__ cnt, u32 b, u64* lds) {
   return block_sum_prefix(cnt, b, lds);
@@ -110,15 +142,18 @@ __device__ __forceinline__ void chunk_mask_scan(const u64* __restrict__ masks, u
     loff[j] = local;
     local += (u32)__popcll(m);
   }
-  lt[tid] = local;
-  __syncthreads();
-  for (u32 d = 1; d < (u32)kBlock; d <<= 1) {
-    const u32 add = tid >= d ? lt[tid - d] : 0u;
-    __syncthreads();
-    lt[tid] += add;
-    __syncthreads();
+  // exclusive prefix of `local` over the block: wave scans, then wave totals
+  const u32 lane = tid & 63u, wave = tid >> 6;
+  u32 incl = local;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const u32 y = __shfl_up(incl, o, 64);
+    if (lane >= (u32)o) incl += y;
   }
-  const u32 excl = lt[tid] - local;
+  if (lane == 63u) lt[wave] = incl;
+  __syncthreads();
+  u32 excl = incl - local;
+  for (u32 w = 0; w < wave; ++w) excl += lt[w];
   for (u32 j = j0; j < j0 + q && j < nw; ++j) loff[j] += excl;
   __syncthreads();
 }
@@ -892,6 +927,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
                                                        const u64* __restrict__ pkills,
                                                        const u64* __restrict__ ppairs, u32 Gp) {
   __shared__ u64 lds[kWaves];
+  __shared__ u64 lds4[4 * kWaves];
   __shared__ u64 lm[kChunkMaxWords];
   __shared__ u32 loff[kChunkMaxWords];
   __shared__ u32 lt[kBlock];
@@ -902,9 +938,9 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
   if (n_entries + n_wl > cap) return;
   const u32 G = chunk_count(n_wl, kListChunk);
   if (blockIdx.x == 0) {
-    const u64 appended = chunk_offset(cnt, G, lds);
-    const u64 kills = block_sum_prefix(ekills, G, lds) + block_sum_prefix(pkills, Gp, lds);
-    const u64 pairs = block_sum_prefix(ppairs, Gp, lds);
+    u64 sums[4];
+    block_sums4(cnt, G, ekills, G, pkills, Gp, ppairs, Gp, lds4, sums);
+    const u64 appended = sums[0], kills = sums[1] + sums[2], pairs = sums[3];
     if (threadIdx.x == 0) {
       st->slot[out_slot].n_entries = n_entries + appended;
       st->slot[out_slot].n_dead = in.n_dead + kills;
