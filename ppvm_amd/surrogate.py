@@ -212,9 +212,12 @@ def run_config5(grid: np.ndarray, h_values: np.ndarray, t_values: np.ndarray, pl
     hi = (query_idx // len(t_values))
     fb_h = np.unique(hi[fb])
     t0 = time.perf_counter()
-    for i in fb_h:
+    for n_done, i in enumerate(fb_h):
         need = int(steps_grid.ravel()[query_idx[(hi == i) & fb]].max())
         ser = propagate(float(h_values[i]), need) if propagate else grid[i, :need + 1]
+        if log and propagate:
+            log(f"fallback {n_done + 1}/{len(fb_h)}: h={float(h_values[i]):.4f}, {need} steps, "
+                f"{time.perf_counter() - t0:.1f}s")
         sel = (hi == i) & fb
         pred[sel] = ser[steps_grid.ravel()[query_idx[sel]]]
     fallback_s = time.perf_counter() - t0
