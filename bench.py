@@ -184,6 +184,11 @@ def run(args) -> dict:
             if nm in pb:
                 kern[nm]["alg_bytes_total"] = float(pb[nm])
                 kern[nm]["achieved_GBs"] = pb[nm] / (kern[nm]["ms_total"] * 1e-3) / 1e9
+                tr = measured_traffic(nm)
+                if tr is not None:
+                    # rocprofv3 FETCH_SIZE + WRITE_SIZE per launch (committed profile of
+                    # this workload) over this run's time per launch
+                    kern[nm]["measured_hbm_GBs"] = tr * int(launches[i]) / (kern[nm]["ms_total"] * 1e-3) / 1e9
     # the dominant kernel: the single kernel with the largest total time (the
     # pairing phase is several kernels, each well below the gather)
     dom = max((k for k in kern if k.startswith("k_") and k in pb), key=lambda k: kern[k]["ms_total"])
