@@ -54,10 +54,11 @@ constexpr int kMaxGrid = 1024;
 constexpr int kOverlapGrid = 512;
 constexpr u32 kGenLimit = (1u << 24) - 2;
 
-enum IssuedKind { IK_ROT = 1, IK_MAP = 2 };
+enum IssuedKind { IK_ROT = 1, IK_MAP = 2, IK_SKIP = 3 };
 struct Issued {
   int kind;
   u64 seq;
+  u32 nskip;  // IK_SKIP: rotations in the run
   RotSpec rot;
   MapProgram map;
 };
@@ -132,6 +133,10 @@ struct ppvm_sum {
   u32 n_preserve = 0;
   std::vector<u64> preserve_host;
   u64 op_seq = 0;
+  // Union of the x (and z) words of every term the sum may hold: exact after
+  // an upload, then widened by every op that can set a bit, never narrowed.
+  std::vector<u64> xsup, zsup;
+  u32 skip_run = 0;  // rotations dropped since the last issued op (see k_skip_gates)
   u32 tab_gen = 0;
   bool clean = true;
   bool broken = false;
@@ -617,8 +622,18 @@ int compile_pattern(const char* text, u32 nq, PatternProg& prog) {
   return PPVM_OK;
 }
 
+int launch_skip(ppvm_sum* s, const Issued& it) {
+  const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
+  int rc;
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_skip_gates, dim3(1), dim3(64), 0, s->stream, s->st, in, out, it.nskip);
+  if ((rc = prof_end(s))) return rc;
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
 int issue(ppvm_sum* s, Issued& it, bool log_it) {
-  int rc = it.kind == IK_ROT ? issue_rot_w(s, it) : issue_map_w(s, it);
+  int rc = it.kind == IK_ROT ? issue_rot_w(s, it) : it.kind == IK_MAP ? issue_map_w(s, it) : launch_skip(s, it);
   if (rc) {
     s->broken = true;
     return rc;
@@ -814,6 +829,96 @@ int issue_and_maybe_sync(ppvm_sum* s, Issued& it) {
   return PPVM_OK;
 }
 
+// ---- support of the sum (host side) ------------------------------------------------
+bool sup_bit(const std::vector<u64>& m, u32 q) { return (m[q >> 6] >> (q & 63)) & 1ull; }
+void sup_put(std::vector<u64>& m, u32 q, bool v) {
+  if (v) m[q >> 6] |= 1ull << (q & 63);
+  else m[q >> 6] &= ~(1ull << (q & 63));
+}
+void sup_from_keys(ppvm_sum* s, const std::vector<u64>& keys, size_t n) {
+  const u32 W = s->W;
+  s->xsup.assign(W, 0ull);
+  s->zsup.assign(W, 0ull);
+  for (size_t i = 0; i < n; ++i)
+    for (u32 p = 0; p < W; ++p) {
+      s->xsup[p] |= keys[i * 2 * W + p];
+      s->zsup[p] |= keys[i * 2 * W + W + p];
+    }
+}
+// Can some term anticommute with the rotation's generator?  A term
+// anticommutes only if it has x (or z) set on a generator qubit where the
+// generator has z (or x), so a clear support bit rules the gate out.
+bool rot_may_act(const ppvm_sum* s, const RotSpec& g) {
+  bool r = (g.gaz && sup_bit(s->xsup, g.a)) || (g.gax && sup_bit(s->zsup, g.a));
+  if (g.two) r = r || (g.gbz && sup_bit(s->xsup, g.b)) || (g.gbx && sup_bit(s->zsup, g.b));
+  return r;
+}
+// Children of a rotation differ from their parents in the generator's bits.
+void sup_after_rot(ppvm_sum* s, const RotSpec& g) {
+  if (g.mode == kBranchDamping) return;  // Z -> I only clears bits
+  if (g.gax) sup_put(s->xsup, g.a, true);
+  if (g.gaz) sup_put(s->zsup, g.a, true);
+  if (g.two) {
+    if (g.gbx) sup_put(s->xsup, g.b, true);
+    if (g.gbz) sup_put(s->zsup, g.b, true);
+  }
+}
+// Clifford bit maps applied to the support (a swap is exact, an XOR becomes an
+// OR); sign-only gates, noise and truncation leave it as it is.
+void sup_after_map(ppvm_sum* s, const ppvm_op& op) {
+  const u32 a = op.q0, b = op.q1;
+  const bool xa = sup_bit(s->xsup, a), za = sup_bit(s->zsup, a);
+  switch (op.code) {
+    case PPVM_OP_H:
+    case PPVM_OP_SQRT_Y:
+    case PPVM_OP_SQRT_Y_DAG:
+      sup_put(s->xsup, a, za);
+      sup_put(s->zsup, a, xa);
+      break;
+    case PPVM_OP_S:
+    case PPVM_OP_S_DAG:
+      if (xa) sup_put(s->zsup, a, true);
+      break;
+    case PPVM_OP_SQRT_X:
+    case PPVM_OP_SQRT_X_DAG:
+      if (za) sup_put(s->xsup, a, true);
+      break;
+    case PPVM_OP_CNOT: {  // z_a ^= z_b, x_b ^= x_a
+      const bool zb = sup_bit(s->zsup, b);
+      if (zb) sup_put(s->zsup, a, true);
+      if (xa) sup_put(s->xsup, b, true);
+      break;
+    }
+    case PPVM_OP_CZ: {  // z_a ^= x_b, z_b ^= x_a
+      const bool xb = sup_bit(s->xsup, b);
+      if (xb) sup_put(s->zsup, a, true);
+      if (xa) sup_put(s->zsup, b, true);
+      break;
+    }
+    case PPVM_OP_CY: {  // z_a ^= x_b ^ z_b, x_b ^= x_a, z_b ^= x_a
+      const bool xb = sup_bit(s->xsup, b), zb = sup_bit(s->zsup, b);
+      if (xb || zb) sup_put(s->zsup, a, true);
+      if (xa) {
+        sup_put(s->xsup, b, true);
+        sup_put(s->zsup, b, true);
+      }
+      break;
+    }
+    default:
+      break;
+  }
+}
+
+// Issues the pending run of dropped rotations (their bookkeeping kernel).
+int flush_skips(ppvm_sum* s) {
+  if (!s->skip_run) return PPVM_OK;
+  Issued it{};
+  it.kind = IK_SKIP;
+  it.nskip = s->skip_run;
+  s->skip_run = 0;
+  return issue_and_maybe_sync(s, it);
+}
+
 // Turns queued ops into kernel launches.  With final=false, a trailing op whose
 // fusion partner may still arrive (a rotation that a truncate could follow, a
 // run of map ops) is kept queued.
@@ -835,7 +940,19 @@ int pump(ppvm_sum* s, bool final) {
       Issued it{};
       it.kind = IK_ROT;
       it.rot = make_rot(op, fused);
+      if (op.code != kOpDampBranch && !rot_may_act(s, it.rot)) {
+        // no term anticommutes with the generator: the gate changes nothing,
+        // and neither does a truncate fused with it on a clean state
+        ++s->skip_run;
+        i += fused ? 2 : 1;
+        if (fused) {
+          while (i < q.size() && q[i].code == PPVM_OP_TRUNCATE) ++i;
+        }
+        continue;
+      }
+      if ((rc = flush_skips(s))) return rc;
       if ((rc = issue_and_maybe_sync(s, it))) return rc;
+      sup_after_rot(s, it.rot);
       i += fused ? 2 : 1;
       if (!fused && s->trunc_active) s->clean = false;
       if (fused) {  // drop repeated truncates (idempotent)
@@ -880,6 +997,7 @@ int pump(ppvm_sum* s, bool final) {
       }
       const bool tnext = k + 1 < q.size() && q[k + 1].code == PPVM_OP_TRUNCATE && s->trunc_active;
       if (!map_add(p, ntables, m, tnext)) break;  // program full: issue and continue
+      sup_after_map(s, m);
       any = true;
       ++k;
       if (tnext) {
@@ -890,13 +1008,14 @@ int pump(ppvm_sum* s, bool final) {
       }
     }
     if (any || p.trunc_before) {
+      if ((rc = flush_skips(s))) return rc;
       if ((rc = issue_and_maybe_sync(s, it))) return rc;
     }
     s->clean = clean;
     i = k;
   }
   q.erase(q.begin(), q.begin() + (long)i);
-  return PPVM_OK;
+  return flush_skips(s);
 }
 
 int enqueue(ppvm_sum* s, const ppvm_op& op) {
@@ -1020,6 +1139,7 @@ int upload(ppvm_sum* s, const std::vector<u64>& keys, const std::vector<double>&
   HIPCHK(hipStreamSynchronize(s->stream));
   s->n_bound = n;
   s->n_synced = s->n_bound;
+  sup_from_keys(s, keys, n);
   return PPVM_OK;
 }
 
@@ -1054,6 +1174,8 @@ int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint,
   s->trunc_active = tau > 0.0 || maxw != ~0ull;
   s->partition_min = partition_min_entries();
   s->bucket_lg_max = bucket_lg_limit();
+  s->xsup.assign(Wp, 0ull);
+  s->zsup.assign(Wp, 0ull);
   s->clean = true;
   HIPCHK(hipGetDevice(&s->device));
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -1180,6 +1302,8 @@ int ppvm_clone(ppvm_handle h, ppvm_handle* out) {
   c->n_bound = cs.n_entries;
   c->n_synced = c->n_bound;
   c->partition_min = h->partition_min;
+  c->xsup = h->xsup;
+  c->zsup = h->zsup;
   c->bucket_lg_max = h->bucket_lg_max;
   c->clean = h->clean;
   c->preserve_host = h->preserve_host;

@@ -341,6 +341,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ p
     }
   }
   for (u32 ch = blockIdx.x; ch < G; ch += gridDim.x) {
+    if (cnt[ch] == 0) continue;  // no anticommuting entry in this chunk (block-uniform)
     u64 lo, hi;
     block_chunk(n, G, ch, lo, hi);
     const u32 nw = (u32)((hi - lo + 63) / 64);
@@ -979,6 +980,21 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
     }
     __syncthreads();  // lm / loff are reused by the next chunk
   }
+}
+
+// ---- k_skip_gates: rotations that act on no term --------------------------------
+// The host drops a run of rotations whose generators anticommute with no Pauli
+// present anywhere in the sum (judged from the union of the terms' x and z
+// supports); such gates leave every term and coefficient unchanged.  This
+// kernel carries the counts over to the next op slot and books the term
+// updates the gates represent.
+__global__ void k_skip_gates(DevState* st, u32 in_slot, u32 out_slot, u32 n_gates) {
+  if (threadIdx.x != 0 || st->overflow) return;
+  const CountSlot in = st->slot[in_slot];
+  st->slot[out_slot] = in;
+  st->term_updates += (u64)n_gates * (in.n_entries - in.n_dead);
+  st->gates += n_gates;
+  st->rotations += n_gates;
 }
 
 // ---- k_finish: single-block reduction / bookkeeping ---------------------------

@@ -158,6 +158,44 @@ def test_clone_is_deep():
     assert_same_terms(c._export(), r.export(), True, "clone")
 
 
+@pytest.mark.parametrize("n,seed,nq_local", [(12, 1, 4), (70, 2, 8), (130, 3, 10)])
+def test_sparse_support_every_gate_kind(n, seed, nq_local):
+    """One weight-1 observable and every gate kind on a few qubits near it, then
+    on all qubits: most rotations act outside the terms' support, which the
+    engine drops without launching the rotation kernels.  Cliffords move the
+    support (H swaps x and z, CNOT/CZ/CY spread it), so a support estimate that
+    is too narrow would drop a rotation that acts and break parity here."""
+    rng = np.random.default_rng(seed)
+    q0 = int(rng.integers(0, n - nq_local))
+    word = ["I"] * n
+    word[q0] = "Z"
+    g, r = _pair(n, ["".join(word)], [1.0], tau=1e-7)
+    local = random_program(rng, nq_local, 150, trunc_every=True)
+    for f in ("q0", "q1"):
+        local[f] += q0
+    wide = random_program(rng, n, 150, trunc_every=True)
+    prog = np.concatenate([local, wide])
+    run_segments(g, r, prog, list(range(9, len(prog), 20)), ctx=f"sparse support n={n}")
+    assert g.stats()["term_updates"] == r.term_updates()
+
+
+def test_rotation_outside_support_is_dropped():
+    """A rotation whose generator meets no term's support launches no rotation
+    kernel (no entry scanned) but still counts its term updates."""
+    from ppvm_amd import PauliSum
+    g = PauliSum(["ZIIII"], 5, min_abs_coeff=1e-9)
+    g.rz(3, theta=0.4)     # Z generator, no X/Y anywhere: commutes with every term
+    g.rzz(2, 4, theta=0.3)
+    c = g._h.counters()
+    assert c["rotations"] == 2 and c["scanned"] == 0 and c["term_updates"] == 2
+    g.rx(0, theta=0.5)     # anticommutes with Z_0: runs
+    c = g._h.counters()
+    assert c["rotations"] == 3 and c["scanned"] == 1 and len(g) == 2
+    g.rz(0, theta=0.2)     # the child Y_0 now has x set on qubit 0: runs
+    c = g._h.counters()
+    assert c["scanned"] == 3 and c["wl_total"] == 2
+
+
 def test_determinism():
     w = WL.truncate_steps(WL.lattice_tfim(), 4)
     a, b = gpu_for(w), gpu_for(w)
