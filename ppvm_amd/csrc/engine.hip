@@ -654,6 +654,47 @@ int read_state(ppvm_sum* s) {
 
 // Synchronise; on overflow grow and replay.  Afterwards the log is empty and
 // st_host holds the state after the last issued op.
+// Exact support of the live terms, read back at a synchronisation; rotations
+// issued later widen it again (sup_after_rot, sup_after_map).  Above this many
+// entries the pass costs about what the dropped gates save, and the support
+// stays the widened estimate.
+constexpr u64 kSupportRefreshMax = 16u << 20;
+
+template <int W>
+int launch_support(ppvm_sum* s, u32 G) {
+  hipLaunchKernelGGL(k_support<W>, dim3(G), dim3(kBlock), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
+                     s->cap, s->st, (u32)(s->op_seq & 1), s->part(5));
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
+int refresh_support(ppvm_sum* s, u64 n) {
+  const u32 W = s->W;
+  s->xsup.assign(W, 0ull);
+  s->zsup.assign(W, 0ull);
+  if (n == 0) return PPVM_OK;
+  const u32 G = (u32)grid_for(n, (u64)kBlock * 8, 1024);  // G * 2W <= kMaxParts
+  int rc;
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
+  switch (W) {
+    case 1: rc = launch_support<1>(s, G); break;
+    case 2: rc = launch_support<2>(s, G); break;
+    case 4: rc = launch_support<4>(s, G); break;
+    default: rc = launch_support<8>(s, G); break;
+  }
+  if (rc) return rc;
+  if ((rc = prof_end(s))) return rc;
+  std::vector<u64> part((size_t)G * 2 * W);
+  HIPCHK(hipMemcpyAsync(part.data(), s->part(5), part.size() * 8, hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (u32 b = 0; b < G; ++b)
+    for (u32 p = 0; p < W; ++p) {
+      s->xsup[p] |= part[(size_t)b * 2 * W + p];
+      s->zsup[p] |= part[(size_t)b * 2 * W + W + p];
+    }
+  return PPVM_OK;
+}
+
 int sync_check(ppvm_sum* s) {
   if (s->broken) return fail(PPVM_ESTATE, "handle is unusable after an earlier device error");
   int rc;
@@ -718,6 +759,10 @@ int sync_check(ppvm_sum* s) {
       return rc;
     }
     s->hs.regrows += 1;
+  }
+  if (c2.n_entries <= kSupportRefreshMax && (rc = refresh_support(s, c2.n_entries))) {
+    s->broken = true;
+    return rc;
   }
   return PPVM_OK;
 }

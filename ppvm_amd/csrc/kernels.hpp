@@ -982,6 +982,40 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
   }
 }
 
+// ---- k_support: union of the live terms' x and z words --------------------------
+// Per block, the OR of every live entry's 2W key words, written to
+// out[block * 2W + p]; the host ORs the blocks.  Lets the host judge exactly
+// which rotations can act on some term (see k_skip_gates).
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_support(const u64* __restrict__ planes,
+                                                    const double* __restrict__ coef, u64 cap,
+                                                    const DevState* __restrict__ st, u32 in_slot,
+                                                    u64* __restrict__ out) {
+  __shared__ u64 lds[2 * W][kWaves];
+  const u64 n = st->slot[in_slot].n_entries;
+  u64 acc[2 * W];
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p) acc[p] = 0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    if (is_dead(coef[i])) continue;
+#pragma unroll
+    for (int p = 0; p < 2 * W; ++p) acc[p] |= planes[(u64)p * cap + i];
+  }
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p) {
+    for (int o = 32; o > 0; o >>= 1) acc[p] |= (u64)__shfl_xor((long long)acc[p], o, 64);
+    if (lane == 0) lds[p][wave] = acc[p];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * W) {
+    u64 v = 0;
+    for (int w = 0; w < kWaves; ++w) v |= lds[threadIdx.x][w];
+    out[(u64)blockIdx.x * (2 * W) + threadIdx.x] = v;
+  }
+}
+
 // ---- k_skip_gates: rotations that act on no term --------------------------------
 // The host drops a run of rotations whose generators anticommute with no Pauli
 // present anywhere in the sum (judged from the union of the terms' x and z
