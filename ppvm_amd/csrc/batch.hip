@@ -103,7 +103,13 @@ struct BInst {
   u64 peak;
   double ovl_last;
   u64 t_done;     // device wall clock when the instance finished (or stopped)
+  // sup_valid: the instance's row of the support array (union of the live
+  // terms' x words [0, W) and z words [W, 2W)) is current; clean: every term
+  // already passed the truncation criteria
+  u32 sup_valid;
+  u32 clean;
 };
+constexpr int kSupWords = 16;  // support words per instance (2W, W <= 8)
 
 constexpr int kBMaxThreads = 1024;
 
@@ -152,9 +158,66 @@ __device__ __forceinline__ void bstore(u64* planes, double* coef, u64 cap, u64 i
   coef[i] = c;
 }
 
+// ---- support of an instance (in LDS while the block works on it) --------------
+// s_sup[0, W): OR of the live terms' x words, s_sup[W, 2W): z words.
+template <int W>
+__device__ __forceinline__ void sup_or(u64 (&acc)[2 * W], const u64 (&k)[2 * W]) {
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p) acc[p] |= k[p];
+}
+// Block OR of every thread's acc into s_sup (lsup: kBMaxThreads/64 x 16 words).
+template <int W>
+__device__ void sup_publish(u64 (&acc)[2 * W], u64* s_sup, u64* lsup) {
+  const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p)
+    for (int o = 32; o > 0; o >>= 1) acc[p] |= (u64)__shfl_xor((long long)acc[p], o, 64);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int p = 0; p < 2 * W; ++p) lsup[wave * 16 + p] = acc[p];
+  }
+  __syncthreads();
+  if (threadIdx.x < 2 * W) {
+    u64 v = 0;
+    for (u32 w = 0; w < nwave; ++w) v |= lsup[w * 16 + threadIdx.x];
+    s_sup[threadIdx.x] = v;
+  }
+  __syncthreads();
+}
+template <int W>
+__device__ void b_support(const BInst& I, u64* s_sup, u64* lsup) {
+  u64 acc[2 * W];
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p) acc[p] = 0;
+  const u64* P = I.planes[I.cur];
+  for (u64 i = threadIdx.x; i < I.n; i += blockDim.x) {
+    u64 k[2 * W];
+    load_key<W>(P, I.cap, i, k);
+    sup_or<W>(acc, k);
+  }
+  sup_publish<W>(acc, s_sup, lsup);
+}
+// Can some live term anticommute with the rotation's generator?  (It needs x
+// where the generator has z, or z where it has x, on a generator qubit.)
+__device__ __forceinline__ bool b_may_act(const u64* s_sup, u32 W, const BOp& op) {
+  u32 ax = 0, az = 0, bx = 0, bz = 0;
+  switch (op.code) {
+    case PPVM_OP_RX: ax = 1; break;
+    case PPVM_OP_RY: ax = az = 1; break;
+    case PPVM_OP_RZ: az = 1; break;
+    case PPVM_OP_RXX: ax = bx = 1; break;
+    case PPVM_OP_RYY: ax = az = bx = bz = 1; break;
+    case PPVM_OP_RZZ: az = bz = 1; break;
+  }
+  auto xb = [&](u32 q) { return (s_sup[q >> 6] >> (q & 63)) & 1ull; };
+  auto zb = [&](u32 q) { return (s_sup[W + (q >> 6)] >> (q & 63)) & 1ull; };
+  return (az && xb(op.a)) || (ax && zb(op.a)) || (bz && xb(op.b)) || (bx && zb(op.b));
+}
+
 // Applies one rotation: arena cur -> cur ^ 1.  Returns false on overflow.
 template <int W>
-__device__ bool b_rotate(BInst& I, const BOp& op, bool trunc, u32* lds) {
+__device__ bool b_rotate(BInst& I, const BOp& op, bool trunc, u32* lds, u64* s_sup, u64* lsup) {
   RotSpec g{};
   g.a = op.a;
   g.b = op.b;
@@ -221,7 +284,10 @@ __device__ bool b_rotate(BInst& I, const BOp& op, bool trunc, u32* lds) {
     }
   }
   __syncthreads();
-  // phase 2: parents, in order
+  // phase 2: parents, in order (the kept keys also rebuild the support)
+  u64 acc[2 * W];
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p) acc[p] = 0;
   u64 out = 0;
   for (u64 base = 0; base < n; base += T) {
     const u64 i = base + threadIdx.x;
@@ -246,7 +312,10 @@ __device__ bool b_rotate(BInst& I, const BOp& op, bool trunc, u32* lds) {
     }
     u32 pos;
     const u32 tot = block_rank(keep, pos, lds);
-    if (keep) bstore<W>(Q, D, cap, out + pos, k, c);
+    if (keep) {
+      bstore<W>(Q, D, cap, out + pos, k, c);
+      sup_or<W>(acc, k);
+    }
     out += tot;
   }
   // phase 3: children of unpaired anticommuting terms, in order
@@ -271,11 +340,15 @@ __device__ bool b_rotate(BInst& I, const BOp& op, bool trunc, u32* lds) {
       overflow = true;
       break;
     }
-    if (keep) bstore<W>(Q, D, cap, out + pos, k, cc);
+    if (keep) {
+      bstore<W>(Q, D, cap, out + pos, k, cc);
+      sup_or<W>(acc, k);
+    }
     out += tot;
   }
   __syncthreads();
   if (overflow) return false;
+  sup_publish<W>(acc, s_sup, lsup);
   I.n = out;
   I.cur ^= 1u;
   return true;
@@ -283,13 +356,16 @@ __device__ bool b_rotate(BInst& I, const BOp& op, bool trunc, u32* lds) {
 
 // Map op (Clifford / noise) and/or truncation: arena cur -> cur ^ 1.
 template <int W>
-__device__ void b_map(BInst& I, const BOp* op, bool trunc, u32* lds) {
+__device__ void b_map(BInst& I, const BOp* op, bool trunc, u32* lds, u64* s_sup, u64* lsup) {
   const u32 T = blockDim.x;
   const u64 n = I.n, cap = I.cap;
   const u64* P = I.planes[I.cur];
   const double* C = I.coef[I.cur];
   u64* Q = I.planes[I.cur ^ 1];
   double* D = I.coef[I.cur ^ 1];
+  u64 acc[2 * W];
+#pragma unroll
+  for (int p = 0; p < 2 * W; ++p) acc[p] = 0;
   u64 out = 0;
   for (u64 base = 0; base < n; base += T) {
     const u64 i = base + threadIdx.x;
@@ -311,10 +387,14 @@ __device__ void b_map(BInst& I, const BOp* op, bool trunc, u32* lds) {
     }
     u32 pos;
     const u32 tot = block_rank(keep, pos, lds);
-    if (keep) bstore<W>(Q, D, cap, out + pos, k, c);
+    if (keep) {
+      bstore<W>(Q, D, cap, out + pos, k, c);
+      sup_or<W>(acc, k);
+    }
     out += tot;
   }
   __syncthreads();
+  sup_publish<W>(acc, s_sup, lsup);
   I.n = out;
   I.cur ^= 1u;
 }
@@ -334,10 +414,12 @@ __device__ double b_overlap(const BInst& I, double* ldsd) {
 }
 
 template <int W>
-__global__ __launch_bounds__(kBMaxThreads) void k_batch(BInst* insts, const u32* order, u32 n_order,
+__global__ __launch_bounds__(kBMaxThreads) void k_batch(BInst* insts, u64* sups, const u32* order, u32 n_order,
                                                         u32* queue, u32 op_lo, u32 op_hi) {
   __shared__ u32 lds[kBMaxThreads / 64 + 1];
   __shared__ double ldsd[kBMaxThreads];
+  __shared__ u64 lsup[(kBMaxThreads / 64) * 16];
+  __shared__ u64 s_sup[kSupWords];
   __shared__ u32 lds_i;
   while (true) {
     if (threadIdx.x == 0) lds_i = add_u32(queue, 1u);
@@ -348,8 +430,16 @@ __global__ __launch_bounds__(kBMaxThreads) void k_batch(BInst* insts, const u32*
     // Every thread keeps a private copy of the instance state; all updates are
     // block-uniform, and thread 0 writes the state back at the end.
     BInst* Ig = &insts[order[qi]];
+    u64* sup_g = sups + (u64)order[qi] * kSupWords;
     BInst I = *Ig;
     if (I.status == IS_OVERFLOW || I.status == IS_DONE) continue;
+    if (threadIdx.x < kSupWords) s_sup[threadIdx.x] = sup_g[threadIdx.x];
+    __syncthreads();
+    if (!I.sup_valid) {
+      b_support<W>(I, s_sup, lsup);
+      I.sup_valid = 1;
+    }
+    const bool trunc_active = I.tau > 0.0 || I.maxw != ~0ull;
     const u32 end = I.n_ops < op_hi ? I.n_ops : op_hi;
     u32 j = I.op_next > op_lo ? I.op_next : op_lo;
     bool ok = true;
@@ -360,14 +450,21 @@ __global__ __launch_bounds__(kBMaxThreads) void k_batch(BInst* insts, const u32*
         // truncation fused into the previous gate
       } else if (op.code >= PPVM_OP_RX && op.code <= PPVM_OP_RZZ) {
         I.term_updates += I.n;
-        I.gen = I.gen + 1u;
-        ok = b_rotate<W>(I, op, trunc, lds);
-        if (!ok) break;
+        // a rotation that anticommutes with no live term changes nothing, and
+        // the truncation fused with it has nothing to drop on a clean state
+        if (b_may_act(s_sup, W, op) || (trunc && !I.clean)) {
+          I.gen = I.gen + 1u;
+          ok = b_rotate<W>(I, op, trunc, lds, s_sup, lsup);
+          if (!ok) break;
+          I.clean = (trunc || !trunc_active) ? 1u : 0u;
+        }
       } else if (op.code == PPVM_OP_TRUNCATE) {
-        b_map<W>(I, nullptr, true, lds);
+        b_map<W>(I, nullptr, true, lds, s_sup, lsup);
+        I.clean = 1u;
       } else if (op.code != PPVM_OP_NOP) {
         I.term_updates += I.n;
-        b_map<W>(I, &op, trunc, lds);
+        b_map<W>(I, &op, trunc, lds, s_sup, lsup);
+        I.clean = (trunc || !trunc_active) ? 1u : 0u;
       }
       if (I.n > I.peak) I.peak = I.n;
       if (op.flags & 2u) {
@@ -392,7 +489,10 @@ __global__ __launch_bounds__(kBMaxThreads) void k_batch(BInst* insts, const u32*
       Ig->peak = I.peak;
       Ig->ovl_last = I.ovl_last;
       Ig->t_done = wall_clock64();
+      Ig->sup_valid = I.sup_valid;
+      Ig->clean = I.clean;
     }
+    if (threadIdx.x < kSupWords) sup_g[threadIdx.x] = s_sup[threadIdx.x];
     __syncthreads();
   }
 }
@@ -413,6 +513,8 @@ __global__ void k_batch_reset(BInst* insts, u32 n, u64* t0) {
   I.peak = I.n_init;
   I.ovl_last = 0.0;
   I.t_done = 0;
+  I.sup_valid = 0;
+  I.clean = 0;
 }
 
 __global__ void k_bfill_u32(u32* p, u32 n, u32 v) {
@@ -433,6 +535,7 @@ struct ppvm_batch {
   u32 n = 0;
   hipStream_t stream = nullptr;
   BInst* d_inst = nullptr;
+  u64* d_sup = nullptr;           // kSupWords per instance (see BInst::sup_valid)
   std::vector<BInst> h_inst;      // host mirror (pointers)
   std::vector<void*> allocs;      // per-instance allocations
   BOp* d_ops = nullptr;
@@ -503,7 +606,7 @@ int pull_instances(ppvm_batch* b) {
 
 template <int W>
 int launch_batch(ppvm_batch* b, u32 threads, u32 blocks, u32 n_order, u32 seg, u32 op_lo, u32 op_hi) {
-  hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(threads), 0, b->stream, b->d_inst, b->d_order, n_order,
+  hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(threads), 0, b->stream, b->d_inst, b->d_sup, b->d_order, n_order,
                      b->d_queue + seg, op_lo, op_hi);
   BCHK(hipGetLastError());
   return PPVM_OK;
@@ -710,6 +813,7 @@ int ppvm_batch_create(const ppvm_batch_instance* inst, size_t n, ppvm_batch_hand
   }
   int rc = PPVM_OK;
   if (hipMalloc(&b->d_inst, n * sizeof(BInst)) != hipSuccess ||
+      hipMalloc(&b->d_sup, (size_t)n * kSupWords * sizeof(u64)) != hipSuccess ||
       hipMalloc(&b->d_ops, std::max<size_t>(1, ops.size()) * sizeof(BOp)) != hipSuccess ||
       hipMalloc(&b->d_factors, std::max<size_t>(1, facs.size()) * sizeof(double)) != hipSuccess ||
       hipMalloc(&b->d_order, n * sizeof(u32)) != hipSuccess ||
@@ -762,6 +866,7 @@ int ppvm_batch_destroy(ppvm_batch_handle b) {
     if (I.mark_ovl) (void)hipFree(I.mark_ovl);
   }
   if (b->d_inst) (void)hipFree(b->d_inst);
+  if (b->d_sup) (void)hipFree(b->d_sup);
   if (b->d_ops) (void)hipFree(b->d_ops);
   if (b->d_factors) (void)hipFree(b->d_factors);
   if (b->d_order) (void)hipFree(b->d_order);
