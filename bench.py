@@ -65,11 +65,13 @@ def phase_bytes(cnt: dict, W: int, nq: int) -> dict:
                    entries tested from the key planes and transposed.
     k_rot_gather : N/8 B masks + N_a (S + R + 8): key and coefficient read,
                    record and pair-key hash written.
-    pairing      : N_a 8 B hashes read + N_p (2 R + 24): both records of each
-                   merged pair read, two coefficients and two flags written
-                   (the minimum of either implementation; the partition's own
-                   traffic shows in the measured bytes).
-    k_emit_count : N_a R records read + (N_a - 2 N_p) 8 B coefficients written + N_a/8 masks.
+    pairing      : N_a 8 B hashes read + N_p (2 R + 32): both records of each
+                   merged pair read, each member's merged coefficient and flag
+                   written back into its record (16 B) (the minimum of either
+                   implementation; the partition's own traffic shows in the
+                   measured bytes).
+    k_emit_count : N_a R records read + N_a 8 B coefficients written (scaled
+                   parents and merged pairs) + N_a/8 masks.
     k_emit_write : N_a/8 masks + N_c (R + S): record read, child term written.
     """
     S = 16 * W + 8
@@ -81,8 +83,8 @@ def phase_bytes(cnt: dict, W: int, nq: int) -> dict:
     return {
         "k_rot_count": cnt["scan_planes"] / 8 + N / 8 + cnt["transposed"] * (16 * W + 2 * nq / 8),
         "k_rot_gather": N / 8 + na * (S + R + 8),
-        "pairing": na * 8 + npair * (2 * R + 24),
-        "k_emit_count": na * R + (na - 2 * npair) * 8 + na / 8,
+        "pairing": na * 8 + npair * (2 * R + 32),
+        "k_emit_count": na * R + na * 8 + na / 8,
         "k_emit_write": na / 8 + nch * (R + S),
     }
 

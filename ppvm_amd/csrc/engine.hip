@@ -403,12 +403,12 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
                        s->cap, lgm, hist, bmeta);
     hipLaunchKernelGGL(k_pair_scatter, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st, in,
                        s->cap, lgm, hist, s->bkt, bmeta);
-    hipLaunchKernelGGL(k_pair_lds<W>, dim3(Gp), dim3(kPairThreads), 0, s->stream, coef, s->cap, wl, s->st, in,
+    hipLaunchKernelGGL(k_pair_lds<W>, dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, wl, s->st, in,
                        it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(), s->part(1),
                        s->part(2));
   } else {
     Gp = (u32)grid_for(ne, kBlock * 2, kFixedGrid);
-    hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, coef, s->cap, wl, s->st, in,
+    hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, s->cap, wl, s->st, in,
                        it.rot, s->hsh, s->table, s->tcap, tg, s->trunc_spec(), it.seq + 1, s->part(1),
                        s->part(2));
   }

@@ -15,7 +15,8 @@
 //                  and writes them in place (with fused truncation).
 //   k_emit_count : unpaired records scale their parent in place and decide
 //                  whether their child k ^ M (coefficient eps_k c_k sin)
-//                  survives the truncation; masks + counts.
+//                  survives the truncation; merged records store their new
+//                  coefficient in the arena; masks + counts.
 //   k_emit_write : append the surviving children at the arena end, in
 //                  worklist order; publish the new counts and statistics.
 // Commuting entries are never touched: their only traffic is the test word.
@@ -418,12 +419,14 @@ __device__ __forceinline__ u32 block_excl_scan_u32(u32 v, u32* wsum) {
 }
 
 // Records i, j whose fingerprints agree: if their pair keys agree too, both
-// merged coefficients are written in place
+// merged coefficients
 //   c_i' = c_i cos + eps_j c_j sin,  c_j' = c_j cos + eps_i c_i sin
 // (damping: the Z member keeps c_Z cos, the I member receives c_I + c_Z sin),
-// with fused truncation, and both records are flagged merged.
+// after fused truncation, go into the records themselves with the merged flag
+// (one 16-byte store each); k_emit_count, which streams the worklist in arena
+// order anyway, moves them into the arena.
 template <int W>
-__device__ __forceinline__ bool pair_try_merge(double* __restrict__ coef, WLRec<W>* __restrict__ wl, u64 i,
+__device__ __forceinline__ bool pair_try_merge(WLRec<W>* __restrict__ wl, u64 i,
                                                u64 j, const RotSpec& g, const TruncSpec& t, u64& kills) {
   u64 k[2 * W], kj[2 * W];
   double ci, cj;
@@ -454,10 +457,9 @@ __device__ __forceinline__ bool pair_try_merge(double* __restrict__ coef, WLRec<
     di = drop_term<W>(k, ni, t);
     dj = drop_term<W>(kj, nj, t);
   }
-  coef[idxi] = di ? dead_value() : ni;
-  coef[idxj] = dj ? dead_value() : nj;
-  wl[i].flags = 1u;
-  wl[j].flags = 1u;
+  const double vi = di ? dead_value() : ni, vj = dj ? dead_value() : nj;
+  reinterpret_cast<ulonglong2*>(wl + i)[W] = make_ulonglong2((u64)__double_as_longlong(vi), (u64)idxi | (1ull << 32));
+  reinterpret_cast<ulonglong2*>(wl + j)[W] = make_ulonglong2((u64)__double_as_longlong(vj), (u64)idxj | (1ull << 32));
   kills += (u64)di + (u64)dj;
   return true;
 }
@@ -466,7 +468,7 @@ __device__ __forceinline__ bool pair_try_merge(double* __restrict__ coef, WLRec<
 // an empty slot (first member of its pair so far) or merges with the member
 // already there.  Returns 1 if it merged.
 template <int W>
-__device__ __forceinline__ u64 pair_global(double* __restrict__ coef, WLRec<W>* __restrict__ wl, u64 i, u64 h,
+__device__ __forceinline__ u64 pair_global(WLRec<W>* __restrict__ wl, u64 i, u64 h,
                                            u64* table, u64 tmask, u32 tgen, const RotSpec& g,
                                            const TruncSpec& t, u64& kills) {
   const u64 hg = mix64(h ^ 0x5bd1e9955bd1e995ull);
@@ -483,7 +485,7 @@ __device__ __forceinline__ u64 pair_global(double* __restrict__ coef, WLRec<W>* 
       sv = old;
       continue;
     }
-    if (((sv >> 32) & 0xffull) == fp && pair_try_merge<W>(coef, wl, i, sv & 0xffffffffull, g, t, kills))
+    if (((sv >> 32) & 0xffull) == fp && pair_try_merge<W>(wl, i, sv & 0xffffffffull, g, t, kills))
       return 1;
     pos = (pos + 1) & tmask;
     sv = table[pos];
@@ -499,7 +501,7 @@ __device__ __forceinline__ u64 pair_table_mask(u64 n_wl, u64 tcap_max) {
 // ---- k_rot_pair -----------------------------------------------------------------
 // Global-table pairing, one record per thread (grid-stride).
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_rot_pair(u64 cap,
                                                      WLRec<W>* __restrict__ wl, DevState* st,
                                                      u32 in_slot, RotSpec g, const u64* __restrict__ hsh,
                                                      u64* table, u64 tcap_max, u32 tgen, TruncSpec t,
@@ -518,7 +520,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(double* __restrict__ coef, 
   const u64 stride = (u64)gridDim.x * kBlock;
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n_wl; i += stride) {
     const u64 h = hsh[i];
-    if (h != kDeadHash) pairs += pair_global<W>(coef, wl, i, h, table, tmask, tgen, g, t, kills);
+    if (h != kDeadHash) pairs += pair_global<W>(wl, i, h, table, tmask, tgen, g, t, kills);
   }
   const u64 bk = block_sum_u64(kills, lds_sum);
   const u64 bp = block_sum_u64(pairs, lds_sum);
@@ -662,7 +664,7 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_scatter(const u64* __rest
 // too large, or a bucket whose positions exceed 20 bits, uses the global
 // table.  Pair members always share bucket and sub-bin.
 template <int W>
-__device__ __forceinline__ u64 lds_probe_sync(double* __restrict__ coef, WLRec<W>* __restrict__ wl,
+__device__ __forceinline__ u64 lds_probe_sync(WLRec<W>* __restrict__ wl,
                                               u32* tab, const u64* __restrict__ bkt, u32 base, u32 e, u64 v,
                                               const RotSpec& g, const TruncSpec& t, u64& kills) {
   const u32 lo = (u32)(v >> 32);
@@ -681,14 +683,14 @@ __device__ __forceinline__ u64 lds_probe_sync(double* __restrict__ coef, WLRec<W
     }
     if ((sv >> 20) == fp && (sv & kPosLimit) != e + 1u) {
       const u32 j = (u32)bkt[base + (sv & kPosLimit) - 1u];
-      if (pair_try_merge<W>(coef, wl, i, j, g, t, kills)) return 1;
+      if (pair_try_merge<W>(wl, i, j, g, t, kills)) return 1;
     }
     pos = (pos + 1u) & (kLdsSlots - 1u);
   }
 }
 
 template <int W>
-__global__ __launch_bounds__(kPairThreads) void k_pair_lds(double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap,
                                                            WLRec<W>* __restrict__ wl, DevState* st,
                                                            u32 in_slot, RotSpec g, u32 lg_max,
                                                            const u64* __restrict__ hsh,
@@ -758,7 +760,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(double* __restrict__ 
       for (u32 c = tid; c < nc; c += kPairThreads) {
         const u32 ce = cand[c] & 0xffffu, cj = cand[c] >> 16;
         const u32 i = (u32)bkt[base + ce], j = (u32)bkt[base + cj];
-        if (pair_try_merge<W>(coef, wl, i, j, g, t, kills)) {
+        if (pair_try_merge<W>(wl, i, j, g, t, kills)) {
           ++pairs;
         } else {
           cand[c] |= 0x80000000u;  // fingerprint collision: walk again below
@@ -769,7 +771,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(double* __restrict__ 
         for (u32 c = tid; c < nc; c += kPairThreads) {
           if (!(cand[c] & 0x80000000u)) continue;
           const u32 ce = cand[c] & 0xffffu;
-          pairs += lds_probe_sync<W>(coef, wl, tab, bkt, base, ce, bkt[base + ce], g, t, kills);
+          pairs += lds_probe_sync<W>(wl, tab, bkt, base, ce, bkt[base + ce], g, t, kills);
         }
       }
       __syncthreads();  // table and candidate list are reset for the next bucket
@@ -818,7 +820,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(double* __restrict__ 
         for (u32 e = tid; e < T; e += kPairThreads) {
           const u64 v = bkt[base + e];
           const u32 sb = (u32)(v >> 58);
-          if (sb >= s0 && sb < s1) pairs += lds_probe_sync<W>(coef, wl, tab, bkt, base, e, v, g, t, kills);
+          if (sb >= s0 && sb < s1) pairs += lds_probe_sync<W>(wl, tab, bkt, base, e, v, g, t, kills);
         }
         __syncthreads();  // the table is cleared again by the next pass
       } else {
@@ -827,7 +829,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(double* __restrict__ 
           const u32 sb = (u32)(v >> 58);
           if (sb >= s0 && sb < s1) {
             const u32 i = (u32)v;
-            pairs += pair_global<W>(coef, wl, i, hsh[i], table, tmask, tgen, g, t, kills);
+            pairs += pair_global<W>(wl, i, hsh[i], table, tmask, tgen, g, t, kills);
             ++fallback;
           }
         }
@@ -883,7 +885,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
       bool keep = false;
       u64 k[2 * W];
       double c = 0.0;
-      u32 idx = 0, fl = 1u;
+      u32 idx = 0, fl = 2u;  // lanes past the chunk end act as dead records
       if (w < hi) load_rec<W>(wl, w, k, c, idx, fl);
       if (fl == 0u) {
         // unpaired: the parent is scaled in place and its child survives
@@ -901,6 +903,8 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
         }
         if (dp || branches) coef[idx] = dp ? dead_value() : cp;
         kills += (u64)dp;
+      } else if (fl == 1u) {
+        coef[idx] = c;  // merged by the pairing kernel (value final, truncation applied)
       }
       const u64 m = __ballot(keep);
       if (lane == 0) masks[(lo >> 6) + j] = m;
