@@ -70,6 +70,9 @@ enum ppvm_opcode {
   PPVM_OP_PAULI_ERROR2 = 43,
   /* amplitude damping (gamma in p[0]), crates/ppvm-pauli-sum/src/sum/noise.rs:132-157 */
   PPVM_OP_AMPLITUDE_DAMPING = 44,
+  /* loss channels, lossy sums only (ppvm_lossy_apply), crates/ppvm-pauli-sum/src/sum/noise.rs:159-274:
+     LOSS (q0, p[0]), CORRELATED_LOSS (q0, q1, p[0..2]), RESET_LOSS (q0) */
+  PPVM_OP_LOSS = 45, PPVM_OP_CORRELATED_LOSS = 46, PPVM_OP_RESET_LOSS = 47,
   /* explicit truncation, crates/ppvm-pauli-sum/src/sum/data.rs:275 */
   PPVM_OP_TRUNCATE = 60
 };
@@ -274,6 +277,46 @@ int ppvm_batch_marks(ppvm_batch_handle b, size_t i, uint64_t* lens, double* over
                      size_t* n);
 int ppvm_batch_export(ppvm_batch_handle b, size_t i, uint64_t* x, uint64_t* z, double* c, size_t cap,
                       size_t* n);
+
+/* ---- lossy sums -----------------------------------------------------------------
+ * Replaces PauliSum<LossyPauliWord> (crates/ppvm-pauli-word/src/loss/data.rs:33-45,
+ * crates/ppvm-python-native/src/interface.rs:13-66, ppvm-python/src/ppvm/paulisum.py:420-520):
+ * Pauli words over {I, X, Y, Z, L}, where L marks a lost qubit.  A word crosses
+ * the boundary as three planes x, z, l of W words each (W as for ppvm_words;
+ * qubit i at bit i % 64 of word i / 64); a lost qubit has l = 1 and x = z = 0.
+ * Gates treat a lost qubit as the reference does (Cliffords and channels skip
+ * the term, a two-qubit rotation acts on the surviving qubit only).  The
+ * truncation strategy is CombinedStrategy(CoefficientThreshold, MaxPauliWeight,
+ * MaxLossWeight) (strategy.rs:12-30,137-170); the Pauli weight counts L.
+ * Error messages are read with ppvm_lossy_last_error. */
+typedef struct ppvm_lossy_sum* ppvm_lossy_handle;
+const char* ppvm_lossy_last_error(void);
+/* max_pauli_weight / max_loss_weight = UINT64_MAX disable those caps. */
+int ppvm_lossy_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_pauli_weight,
+                      uint64_t max_loss_weight, uint64_t capacity_hint, ppvm_lossy_handle* out);
+int ppvm_lossy_destroy(ppvm_lossy_handle h);
+int ppvm_lossy_clone(ppvm_lossy_handle h, ppvm_lossy_handle* out);
+int ppvm_lossy_words(ppvm_lossy_handle h, uint32_t* w);
+/* Preserve set (sum/data.rs:75-77, restore order of sum/data.rs:275-311). */
+int ppvm_lossy_set_preserve(ppvm_lossy_handle h, const uint64_t* x, const uint64_t* z, const uint64_t* l,
+                            size_t count);
+/* `ps += (word, c)` per term in order (sum/ops.rs:140-151). */
+int ppvm_lossy_add_terms(ppvm_lossy_handle h, const uint64_t* x, const uint64_t* z, const uint64_t* l,
+                         const double* c, size_t count);
+/* A program of ppvm_op records: every op code of ppvm_apply plus PPVM_OP_LOSS,
+ * PPVM_OP_CORRELATED_LOSS and PPVM_OP_RESET_LOSS.  Gates never truncate by
+ * themselves; PPVM_OP_TRUNCATE replaces PauliSum::truncate. */
+int ppvm_lossy_apply(ppvm_lossy_handle h, const ppvm_op* ops, size_t n);
+int ppvm_lossy_len(ppvm_lossy_handle h, uint64_t* out);
+/* Terms in insertion order; cap < len writes nothing, sets *n = len, PPVM_EINVAL. */
+int ppvm_lossy_export_terms(ppvm_lossy_handle h, uint64_t* x, uint64_t* z, uint64_t* l, double* c,
+                            size_t cap, size_t* n);
+/* trace("Z?*"): terms with every qubit I or Z (an L never matches a pattern,
+ * pattern/contains.rs:120-151). */
+int ppvm_lossy_overlap_with_zero(ppvm_lossy_handle h, double* out);
+int ppvm_lossy_trace(ppvm_lossy_handle h, const char* pattern, double* out);
+/* sum over terms k of b of b[k] * a[k] (sum/trace.rs:30-37). */
+int ppvm_lossy_overlap(ppvm_lossy_handle a, ppvm_lossy_handle b, double* out);
 
 /* ---- learned surrogate ---------------------------------------------------------
  * No counterpart in the reference.  An ensemble of MLPs (in -> H -> H -> H -> 1,

@@ -54,6 +54,8 @@ struct Sum {
   uint32_t n = 0, W = 1, NW = 2;  // qubits, words per plane, words per key (2W)
   double tau = 0.0;                // CoefficientThreshold
   uint64_t maxw = UINT64_MAX;      // MaxPauliWeight (UINT64_MAX disables)
+  bool lossy = false;              // LossyPauliWord keys: NW = 3W (x, z, loss planes)
+  uint64_t maxlw = UINT64_MAX;     // MaxLossWeight (lossy sums only)
   std::vector<uint64_t> keys;      // entries * NW (x words then z words)
   std::vector<double> coef;
   std::vector<uint64_t> hashes;
@@ -129,6 +131,7 @@ struct Sum {
     uint64_t m = 1ull << (q & 63);
     k[W + (q >> 6)] = v ? (k[W + (q >> 6)] | m) : (k[W + (q >> 6)] & ~m);
   }
+  bool lb(const uint64_t* k, uint32_t q) const { return lossy && ((k[2 * W + (q >> 6)] >> (q & 63)) & 1); }
 };
 
 // ---- Clifford conjugation (sum/clifford.rs; bit maps traits/clifford.rs;
@@ -465,6 +468,8 @@ double overlap_zero(const Sum& s) {
     const uint64_t* k = s.key(i);
     bool zero = true;
     for (uint32_t w = 0; w < s.W; ++w) zero &= (k[w] == 0);
+    if (s.lossy)  // an L is neither I nor Z
+      for (uint32_t w = 0; w < s.W; ++w) zero &= (k[2 * s.W + w] == 0);
     if (zero) acc += s.coef[i];
   }
   return acc;
@@ -577,7 +582,12 @@ bool parse_pattern(const char* text, std::vector<PatItem>& out) {
 bool pattern_contains(const Sum& s, const uint64_t* k, const std::vector<PatItem>& pat) {
   const uint64_t n = s.n;
   uint64_t cur = 0;  // next qubit the word iterator yields (word/data.rs:421-428)
-  auto code = [&](uint64_t q) -> unsigned { return (unsigned)s.xb(k, (uint32_t)q) | ((unsigned)s.zb(k, (uint32_t)q) << 1); };
+  // Pauli of a slot: I 0, X 1, Z 2, Y 3, and L 4 on a lossy word, which no
+  // op pattern accepts and which is not the identity
+  auto code = [&](uint64_t q) -> unsigned {
+    if (s.lb(k, (uint32_t)q)) return 4u;
+    return (unsigned)s.xb(k, (uint32_t)q) | ((unsigned)s.zb(k, (uint32_t)q) << 1);
+  };
   for (const PatItem& it : pat) {
     bool next = false;
     if (it.kind == PK_POSITION) {  // match_position, contains.rs:48-70
@@ -792,6 +802,308 @@ int apply_sharded(Sum& src, const ppvm_op* ops, size_t nops, int threads) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Lossy sums: PauliSum<LossyPauliWord> (crates/ppvm-pauli-word/src/loss/data.rs).
+// Keys are 3W words (x plane, z plane, loss plane).  Each op is the reference
+// closure written out per term:
+//   sum/clifford.rs:37-217 and phase/clifford.rs:16-180 (a lost qubit leaves
+//     the word and sign unchanged),
+//   sum/rot1.rs:9-128, sum/rot2.rs:17-184 (a lost qubit: no rotation, or the
+//     one-qubit rotation on the surviving qubit),
+//   sum/noise.rs:17-274 (channels; loss, correlated loss, reset loss),
+// with the scratch-vector merge of map_insert / map_insert_multiple
+// (sum/data.rs:181-226, map/hashmap.rs:170-179,242-259).
+enum LPauli { LP_I = 0, LP_X = 1, LP_Z = 2, LP_Y = 3, LP_L = 4 };
+
+// LossyPauliWord::get / set (loss/data.rs:232-316)
+int lget(const Sum& s, const uint64_t* k, uint32_t q) {
+  if (s.lb(k, q)) return LP_L;
+  return (int)s.xb(k, q) | ((int)s.zb(k, q) << 1);
+}
+void lset(const Sum& s, uint64_t* k, uint32_t q, int p) {
+  s.setx(k, q, p == LP_X || p == LP_Y);
+  s.setz(k, q, p == LP_Z || p == LP_Y);
+  const uint64_t m = 1ull << (q & 63);
+  uint64_t& w = k[2 * s.W + (q >> 6)];
+  w = p == LP_L ? (w | m) : (w & ~m);
+}
+
+// map_insert_multiple: f(key, &coef, kids, kid_coefs) returns the number of
+// children (0..3); children are merged with `+=` after the pass, in order.
+template <class F>
+void lmap_insert(Sum& s, F f) {
+  std::vector<uint64_t> ck;
+  std::vector<double> cc;
+  std::vector<uint64_t> kids(3 * s.NW);
+  double kv[3];
+  const size_t n = s.len();
+  s.term_updates += n;
+  for (size_t i = 0; i < n; ++i) {
+    const int m = f(s.key(i), s.coef[i], kids.data(), kv);
+    for (int j = 0; j < m; ++j) {
+      ck.insert(ck.end(), kids.begin() + j * s.NW, kids.begin() + (j + 1) * s.NW);
+      cc.push_back(kv[j]);
+    }
+  }
+  for (size_t j = 0; j < cc.size(); ++j) s.add_assign(&ck[j * s.NW], cc[j]);
+}
+
+// rotate_1_map_insert_closure (rot1.rs:9-41) on a possibly lost qubit
+int lrot1(const Sum& s, uint32_t code1, uint32_t q, const uint64_t* k, double& v, double sn, double cs,
+          uint64_t* kid, double* kv) {
+  if (s.lb(k, q)) return 0;
+  const int eps = rot_child(s, code1, q, 0, k, kid);
+  if (eps == 0) return 0;
+  kv[0] = v * ((double)eps * sn);
+  v *= cs;
+  return 1;
+}
+
+void ltruncate(Sum& s) {
+  std::vector<uint64_t> saved_k;
+  std::vector<double> saved_c;
+  const size_t P = s.preserve.size() / s.NW;
+  for (size_t i = 0; i < s.len() && P; ++i)
+    for (size_t p = 0; p < P; ++p)
+      if (std::memcmp(s.key(i), &s.preserve[p * s.NW], s.NW * 8) == 0) {
+        saved_k.insert(saved_k.end(), s.key(i), s.key(i) + s.NW);
+        saved_c.push_back(s.coef[i]);
+      }
+  // CombinedStrategy(CombinedStrategy(CoefficientThreshold, MaxPauliWeight), MaxLossWeight)
+  // (ppvm-python-native/src/interface.rs:58-65; strategy.rs:19-30,86-89,133,160-169)
+  auto retain = [&](auto keep) {
+    size_t o = 0;
+    for (size_t i = 0; i < s.len(); ++i) {
+      if (!keep(i)) continue;
+      if (o != i) {
+        std::memmove(s.key(o), s.key(i), s.NW * 8);
+        s.coef[o] = s.coef[i];
+        s.hashes[o] = s.hashes[i];
+      }
+      ++o;
+    }
+    s.keys.resize(o * s.NW);
+    s.coef.resize(o);
+    s.hashes.resize(o);
+  };
+  retain([&](size_t i) { return !(std::fabs(s.coef[i]) < s.tau); });
+  if (s.maxw != UINT64_MAX)
+    retain([&](size_t i) {  // weight counts X, Y, Z and L (loss/data.rs:134-176)
+      uint64_t w = 0;
+      const uint64_t* k = s.key(i);
+      for (uint32_t p = 0; p < s.W; ++p) w += __builtin_popcountll(k[p] | k[s.W + p] | k[2 * s.W + p]);
+      return w <= s.maxw;
+    });
+  if (s.maxlw != UINT64_MAX)
+    retain([&](size_t i) {  // loss_weight (loss/data.rs:178-203)
+      uint64_t w = 0;
+      const uint64_t* k = s.key(i);
+      for (uint32_t p = 0; p < s.W; ++p) w += __builtin_popcountll(k[2 * s.W + p]);
+      return w <= s.maxlw;
+    });
+  s.rebuild_index();
+  for (size_t j = 0; j < saved_c.size(); ++j) {  // sum/data.rs:305-310
+    const uint64_t* k = &saved_k[j * s.NW];
+    if (s.find(k, key_hash(k, s.NW)) < 0) s.add_assign(k, saved_c[j]);
+  }
+}
+
+int lossy_apply_one(Sum& s, const ppvm_op& op) {
+  const uint32_t code = op.code, a = op.q0, b = op.q1;
+  auto qok = [&](uint32_t q) { return q < s.n; };
+  const bool two = code == PPVM_OP_CNOT || code == PPVM_OP_CZ || code == PPVM_OP_CY || code == PPVM_OP_RXX ||
+                   code == PPVM_OP_RYY || code == PPVM_OP_RZZ || code == PPVM_OP_DEPOLARIZE2 ||
+                   code == PPVM_OP_PAULI_ERROR2 || code == PPVM_OP_CORRELATED_LOSS;
+  if (code == PPVM_OP_NOP) return 0;
+  if (code == PPVM_OP_TRUNCATE) {
+    ltruncate(s);
+    return 0;
+  }
+  if (!qok(a) || (two && (!qok(b) || a == b))) return 1;
+  const uint32_t NW = s.NW;
+  switch (code) {
+    case PPVM_OP_X: case PPVM_OP_Y: case PPVM_OP_Z:  // scale (clifford.rs:37-66)
+      s.term_updates += s.len();
+      for (size_t i = 0; i < s.len(); ++i)
+        if (!s.lb(s.key(i), a) && clifford_word(s, code, a, 0, s.key(i))) s.coef[i] *= -1.0;
+      return 0;
+    case PPVM_OP_H: case PPVM_OP_S: case PPVM_OP_S_DAG: case PPVM_OP_SQRT_X: case PPVM_OP_SQRT_X_DAG:
+    case PPVM_OP_SQRT_Y: case PPVM_OP_SQRT_Y_DAG:  // map_add (clifford.rs:69-214)
+      s.term_updates += s.len();
+      for (size_t i = 0; i < s.len(); ++i) {
+        if (s.lb(s.key(i), a)) continue;
+        if (clifford_word(s, code, a, 0, s.key(i))) s.coef[i] = -s.coef[i];
+      }
+      s.rehash_all();
+      return 0;
+    case PPVM_OP_CNOT: case PPVM_OP_CZ: case PPVM_OP_CY:  // map_word! (clifford.rs:13-31)
+      s.term_updates += s.len();
+      for (size_t i = 0; i < s.len(); ++i) {
+        if (s.lb(s.key(i), a) || s.lb(s.key(i), b)) continue;  // phase/clifford.rs:77,92,169
+        if (clifford_word(s, code, a, b, s.key(i))) s.coef[i] = -s.coef[i];
+      }
+      s.rehash_all();
+      return 0;
+    case PPVM_OP_RX: case PPVM_OP_RY: case PPVM_OP_RZ: {
+      double sn, cs;
+      sincos(op.theta, &sn, &cs);
+      lmap_insert(s, [&](const uint64_t* k, double& v, uint64_t* kid, double* kv) {
+        return lrot1(s, code, a, k, v, sn, cs, kid, kv);
+      });
+      return 0;
+    }
+    case PPVM_OP_RXX: case PPVM_OP_RYY: case PPVM_OP_RZZ: {
+      double sn, cs;
+      sincos(op.theta, &sn, &cs);
+      const uint32_t one = code == PPVM_OP_RXX ? PPVM_OP_RX : code == PPVM_OP_RYY ? PPVM_OP_RY : PPVM_OP_RZ;
+      lmap_insert(s, [&](const uint64_t* k, double& v, uint64_t* kid, double* kv) {
+        if (s.lb(k, a)) return lrot1(s, one, b, k, v, sn, cs, kid, kv);  // rot2.rs:28-35
+        if (s.lb(k, b)) return lrot1(s, one, a, k, v, sn, cs, kid, kv);
+        const int eps = rot_child(s, code, a, b, k, kid);
+        if (eps == 0) return 0;
+        kv[0] = v * ((double)eps * sn);
+        v *= cs;
+        return 1;
+      });
+      return 0;
+    }
+    case PPVM_OP_PAULI_ERROR: {  // noise.rs:17-39
+      s.term_updates += s.len();
+      const double xf = 1.0 - op.p[1] * 2.0 - op.p[2] * 2.0;
+      const double zf = 1.0 - op.p[0] * 2.0 - op.p[1] * 2.0;
+      const double yf = 1.0 - op.p[0] * 2.0 - op.p[2] * 2.0;
+      for (size_t i = 0; i < s.len(); ++i) {
+        const int p = lget(s, s.key(i), a);
+        if (p == LP_X) s.coef[i] *= xf;
+        else if (p == LP_Z) s.coef[i] *= zf;
+        else if (p == LP_Y) s.coef[i] *= yf;
+      }
+      return 0;
+    }
+    case PPVM_OP_DEPOLARIZE1: {  // noise.rs:109-118
+      s.term_updates += s.len();
+      const double f = 1.0 - op.p[0] * (4.0 / 3.0);
+      for (size_t i = 0; i < s.len(); ++i) {
+        const int p = lget(s, s.key(i), a);
+        if (p != LP_L && p != LP_I) s.coef[i] *= f;
+      }
+      return 0;
+    }
+    case PPVM_OP_DEPOLARIZE2: {  // noise.rs:120-131
+      s.term_updates += s.len();
+      const double f = 1.0 - op.p[0] * (16.0 / 15.0);
+      for (size_t i = 0; i < s.len(); ++i) {
+        const int pa = lget(s, s.key(i), a), pb = lget(s, s.key(i), b);
+        if (pa == LP_L || pb == LP_L) continue;
+        if (pa != LP_I || pb != LP_I) s.coef[i] *= f;
+      }
+      return 0;
+    }
+    case PPVM_OP_PAULI_ERROR2: {  // noise.rs:52-107: any L -> no noise
+      s.term_updates += s.len();
+      double fac[16];
+      bool app[16];
+      pe2_factors(op.p, fac, app);
+      for (size_t i = 0; i < s.len(); ++i) {
+        const uint64_t* k = s.key(i);
+        if (s.lb(k, a) || s.lb(k, b)) continue;
+        const int r = 4 * letter(s.xb(k, a), s.zb(k, a)) + letter(s.xb(k, b), s.zb(k, b));
+        if (app[r]) s.coef[i] *= fac[r];
+      }
+      return 0;
+    }
+    case PPVM_OP_AMPLITUDE_DAMPING: {  // noise.rs:132-157
+      const double g = op.p[0];
+      lmap_insert(s, [&](const uint64_t* k, double& v, uint64_t* kid, double* kv) {
+        const int p = lget(s, k, a);
+        if (p == LP_X || p == LP_Y) {
+          v *= std::sqrt(1.0 - g);
+          return 0;
+        }
+        if (p != LP_Z) return 0;
+        kv[0] = v * g;
+        std::memcpy(kid, k, NW * 8);
+        lset(s, kid, a, LP_I);
+        v *= 1.0 - g;
+        return 1;
+      });
+      return 0;
+    }
+    case PPVM_OP_LOSS: {  // noise.rs:165-180
+      const double p = op.p[0];
+      lmap_insert(s, [&](const uint64_t* k, double& v, uint64_t* kid, double* kv) {
+        if (lget(s, k, a) == LP_L) {
+          kv[0] = v * p;
+          std::memcpy(kid, k, NW * 8);
+          lset(s, kid, a, LP_I);
+          return 1;
+        }
+        v *= 1.0 - p;
+        return 0;
+      });
+      return 0;
+    }
+    case PPVM_OP_RESET_LOSS: {  // noise.rs:254-274
+      lmap_insert(s, [&](const uint64_t* k, double& v, uint64_t* kid, double* kv) {
+        const int p = lget(s, k, a);
+        if (p == LP_L) {
+          v *= 0.0;
+          return 0;
+        }
+        if (p == LP_I || p == LP_Z) {
+          std::memcpy(kid, k, NW * 8);
+          lset(s, kid, a, LP_L);
+          kv[0] = v;
+          return 1;
+        }
+        return 0;
+      });
+      return 0;
+    }
+    case PPVM_OP_CORRELATED_LOSS: {  // noise.rs:182-248
+      const double* p = op.p;
+      lmap_insert(s, [&](const uint64_t* k, double& v, uint64_t* kid, double* kv) {
+        const int pa = lget(s, k, a), pb = lget(s, k, b);
+        if (pa == LP_L && pb == LP_L) {
+          const double v_il = v * p[2];
+          std::memcpy(kid, k, NW * 8);  // (I, L)
+          lset(s, kid, a, LP_I);
+          lset(s, kid, b, LP_L);
+          std::memcpy(kid + NW, k, NW * 8);  // (L, I)
+          lset(s, kid + NW, a, LP_L);
+          lset(s, kid + NW, b, LP_I);
+          const double v_ii = v * p[0];
+          std::memcpy(kid + 2 * NW, k, NW * 8);  // (I, I)
+          lset(s, kid + 2 * NW, a, LP_I);
+          lset(s, kid + 2 * NW, b, LP_I);
+          kv[0] = v_il;
+          kv[1] = v_il;
+          kv[2] = v_ii;
+          return 3;
+        }
+        if (pb == LP_L) {
+          std::memcpy(kid, k, NW * 8);
+          lset(s, kid, b, LP_I);
+          kv[0] = v * p[1];
+          v *= 1.0 - p[2];
+          return 1;
+        }
+        if (pa == LP_L) {
+          std::memcpy(kid, k, NW * 8);
+          lset(s, kid, a, LP_I);
+          kv[0] = v * p[1];
+          v *= 1.0 - p[2];
+          return 1;
+        }
+        v *= 1.0 - p[1] * 2.0 - p[0];
+        return 0;
+      });
+      return 0;
+    }
+  }
+  return 1;
+}
+
 }  // namespace
 
 struct ppvo_sum : Sum {};
@@ -830,7 +1142,7 @@ void ppvo_add_terms(ppvo_handle h, const uint64_t* x, const uint64_t* z, const d
 }
 int ppvo_apply(ppvo_handle h, const ppvm_op* ops, size_t n) {
   for (size_t i = 0; i < n; ++i)
-    if (apply_one(*h, ops[i])) return 1;
+    if (h->lossy ? lossy_apply_one(*h, ops[i]) : apply_one(*h, ops[i])) return 1;
   return 0;
 }
 double ppvo_overlap_with_zero(ppvo_handle h) { return overlap_zero(*h); }
@@ -874,6 +1186,45 @@ uint64_t ppvo_export(ppvo_handle h, uint64_t* x, uint64_t* z, double* c, size_t 
   return h->len();
 }
 uint64_t ppvo_term_updates(ppvo_handle h) { return h->term_updates; }
+
+ppvo_handle ppvo_lossy_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_pauli_weight,
+                              uint64_t max_loss_weight) {
+  ppvo_sum* s = ppvo_create(n_qubits, min_abs_coeff, max_pauli_weight);
+  s->lossy = true;
+  s->maxlw = max_loss_weight;
+  s->NW = 3 * s->W;
+  s->rebuild_index();
+  return s;
+}
+static void lossy_key(ppvo_handle h, const uint64_t* x, const uint64_t* z, const uint64_t* l, size_t i,
+                      uint64_t* k) {
+  std::memcpy(k, x + i * h->W, h->W * 8);
+  std::memcpy(k + h->W, z + i * h->W, h->W * 8);
+  std::memcpy(k + 2 * h->W, l + i * h->W, h->W * 8);
+}
+void ppvo_lossy_set_preserve(ppvo_handle h, const uint64_t* x, const uint64_t* z, const uint64_t* l,
+                             size_t count) {
+  h->preserve.assign(count * h->NW, 0);
+  for (size_t i = 0; i < count; ++i) lossy_key(h, x, z, l, i, &h->preserve[i * h->NW]);
+}
+void ppvo_lossy_add_terms(ppvo_handle h, const uint64_t* x, const uint64_t* z, const uint64_t* l,
+                          const double* c, size_t count) {
+  std::vector<uint64_t> k(h->NW);
+  for (size_t i = 0; i < count; ++i) {
+    lossy_key(h, x, z, l, i, k.data());
+    h->add_assign(k.data(), c[i]);
+  }
+}
+uint64_t ppvo_lossy_export(ppvo_handle h, uint64_t* x, uint64_t* z, uint64_t* l, double* c, size_t cap) {
+  size_t n = std::min(cap, h->len());
+  for (size_t i = 0; i < n; ++i) {
+    std::memcpy(x + i * h->W, h->key(i), h->W * 8);
+    std::memcpy(z + i * h->W, h->key(i) + h->W, h->W * 8);
+    std::memcpy(l + i * h->W, h->key(i) + 2 * h->W, h->W * 8);
+    c[i] = h->coef[i];
+  }
+  return h->len();
+}
 size_t ppvo_run_marked(ppvo_handle h, const ppvm_op* ops, size_t n, const uint64_t* marks,
                        size_t n_marks, uint64_t* out_len, double* out_ovl) {
   size_t m = 0;

@@ -59,6 +59,18 @@ size_t ppvo_run_marked(ppvo_handle h, const ppvm_op* ops, size_t n, const uint64
  * path; only the order of the entries differs. */
 int ppvo_apply_sharded(ppvo_handle h, const ppvm_op* ops, size_t n, int threads);
 
+/* Lossy sums (PauliSum<LossyPauliWord>): words carry a loss plane l besides
+ * x and z (W words each).  ppvo_apply, ppvo_len, ppvo_overlap_with_zero,
+ * ppvo_trace and ppvo_overlap accept lossy handles; ppvo_apply takes the loss
+ * channel op codes on them. */
+ppvo_handle ppvo_lossy_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_pauli_weight,
+                              uint64_t max_loss_weight);
+void ppvo_lossy_set_preserve(ppvo_handle h, const uint64_t* x, const uint64_t* z, const uint64_t* l,
+                             size_t count);
+void ppvo_lossy_add_terms(ppvo_handle h, const uint64_t* x, const uint64_t* z, const uint64_t* l,
+                          const double* c, size_t count);
+uint64_t ppvo_lossy_export(ppvo_handle h, uint64_t* x, uint64_t* z, uint64_t* l, double* c, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

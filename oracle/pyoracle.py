@@ -13,7 +13,8 @@ import subprocess
 
 import numpy as np
 
-from ppvm_amd.ops import OP_DTYPE, pack_strings, unpack_words, words_for
+from ppvm_amd.ops import (OP_DTYPE, pack_lossy_strings, pack_strings, unpack_lossy_words, unpack_words,
+                          words_for)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "libppvm_oracle.so")
@@ -64,6 +65,12 @@ def lib():
         L.ppvo_run_marked.restype = ctypes.c_size_t
         L.ppvo_run_marked.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _u64p,
                                       ctypes.c_size_t, _u64p, _f64p]
+        L.ppvo_lossy_create.restype = ctypes.c_void_p
+        L.ppvo_lossy_create.argtypes = [ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64]
+        L.ppvo_lossy_set_preserve.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, ctypes.c_size_t]
+        L.ppvo_lossy_add_terms.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, _f64p, ctypes.c_size_t]
+        L.ppvo_lossy_export.restype = ctypes.c_uint64
+        L.ppvo_lossy_export.argtypes = [ctypes.c_void_p, _u64p, _u64p, _u64p, _f64p, ctypes.c_size_t]
         L.ppvo_apply_sharded.restype = ctypes.c_int
         L.ppvo_apply_sharded.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                          ctypes.c_int]
@@ -175,3 +182,49 @@ class OracleSum:
     def terms(self):
         x, z, c = self.export()
         return list(zip(unpack_words(x, z, self.n_qubits), c.tolist()))
+
+
+class LossyOracleSum(OracleSum):
+    """CPU oracle of a PauliSum over lossy words (I X Y Z L)."""
+
+    def __init__(self, n_qubits: int, min_abs_coeff: float = 0.0,
+                 max_pauli_weight: int | None = None, max_loss_weight: int | None = None, _handle=None):
+        mw = (1 << 64) - 1 if max_pauli_weight is None else int(max_pauli_weight)
+        mlw = (1 << 64) - 1 if max_loss_weight is None else int(max_loss_weight)
+        h = _handle if _handle is not None else lib().ppvo_lossy_create(n_qubits, float(min_abs_coeff), mw, mlw)
+        super().__init__(n_qubits, min_abs_coeff, max_pauli_weight, _handle=h)
+        self.max_loss_weight = max_loss_weight
+
+    def copy(self) -> "LossyOracleSum":
+        return LossyOracleSum(self.n_qubits, self.min_abs_coeff, self.max_pauli_weight,
+                              self.max_loss_weight, _handle=lib().ppvo_clone(self._h))
+
+    def add_lossy_words(self, x, z, l, c):
+        x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, self.W)
+        z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, self.W)
+        l = np.ascontiguousarray(l, dtype=np.uint64).reshape(-1, self.W)
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
+        lib().ppvo_lossy_add_terms(self._h, _p64(x), _p64(z), _p64(l), c.ctypes.data_as(_f64p), len(c))
+
+    def add_strings(self, strings, coeffs=None):
+        x, z, l = pack_lossy_strings(list(strings), self.n_qubits)
+        c = np.ones(len(strings)) if coeffs is None else np.asarray(coeffs, dtype=np.float64)
+        self.add_lossy_words(x, z, l, c)
+
+    def set_preserve(self, strings):
+        x, z, l = pack_lossy_strings(list(strings), self.n_qubits)
+        lib().ppvo_lossy_set_preserve(self._h, _p64(x), _p64(z), _p64(l), len(strings))
+
+    def export(self):
+        n = len(self)
+        x = np.zeros((n, self.W), dtype=np.uint64)
+        z = np.zeros((n, self.W), dtype=np.uint64)
+        l = np.zeros((n, self.W), dtype=np.uint64)
+        c = np.zeros(n, dtype=np.float64)
+        if n:
+            lib().ppvo_lossy_export(self._h, _p64(x), _p64(z), _p64(l), c.ctypes.data_as(_f64p), n)
+        return x, z, l, c
+
+    def terms(self):
+        x, z, l, c = self.export()
+        return list(zip(unpack_lossy_words(x, z, l, self.n_qubits), c.tolist()))

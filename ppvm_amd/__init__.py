@@ -2,6 +2,6 @@
 QuEraComputing/ppvm.  See include/ppvm_hip.h for the C ABI."""
 
 from .ops import Program  # noqa: F401
-from .paulisum import PauliSum  # noqa: F401
+from .paulisum import LossyPauliSum, PauliSum  # noqa: F401
 
-__all__ = ["PauliSum", "Program"]
+__all__ = ["LossyPauliSum", "PauliSum", "Program"]

@@ -11,6 +11,7 @@ import os
 
 import numpy as np
 
+from . import ops as _ops
 from .ops import OP_DTYPE
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -80,6 +81,20 @@ def _declare(L):
     d("ppvm_get_counters", _H, _u64p)
     d("ppvm_profile_enable", _H, ctypes.c_int)
     d("ppvm_profile_read", _H, _f64p, _u64p)
+    d("ppvm_lossy_last_error", res=ctypes.c_char_p)
+    d("ppvm_lossy_create", ctypes.c_uint32, ctypes.c_double, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+      ctypes.POINTER(_H))
+    d("ppvm_lossy_destroy", _H)
+    d("ppvm_lossy_clone", _H, ctypes.POINTER(_H))
+    d("ppvm_lossy_words", _H, _u32p)
+    d("ppvm_lossy_set_preserve", _H, _u64p, _u64p, _u64p, ctypes.c_size_t)
+    d("ppvm_lossy_add_terms", _H, _u64p, _u64p, _u64p, _f64p, ctypes.c_size_t)
+    d("ppvm_lossy_apply", _H, ctypes.c_void_p, ctypes.c_size_t)
+    d("ppvm_lossy_len", _H, _u64p)
+    d("ppvm_lossy_export_terms", _H, _u64p, _u64p, _u64p, _f64p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t))
+    d("ppvm_lossy_overlap_with_zero", _H, _f64p)
+    d("ppvm_lossy_trace", _H, ctypes.c_char_p, _f64p)
+    d("ppvm_lossy_overlap", _H, _H, _f64p)
 
 
 def lib():
@@ -96,9 +111,9 @@ def lib():
     return _lib
 
 
-def check(rc: int) -> None:
+def check(rc: int, err=None) -> None:
     if rc != PPVM_OK:
-        msg = lib().ppvm_last_error().decode("utf-8", "replace")
+        msg = (err or lib().ppvm_last_error)().decode("utf-8", "replace")
         if rc == PPVM_EINVAL:
             raise ValueError(msg)
         if rc == PPVM_ENOMEM:
@@ -188,6 +203,29 @@ class Handle:
         program = np.ascontiguousarray(program, dtype=OP_DTYPE)
         check(lib().ppvm_apply(self.h, program.ctypes.data, len(program)))
 
+    def r(self, q: int, axis_angle: float, theta: float):
+        check(lib().ppvm_r(self.h, int(q), float(axis_angle), float(theta)))
+
+    def pauli_error(self, q: int, p):
+        arr = np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(3))
+        check(lib().ppvm_pauli_error(self.h, int(q), f64p(arr)))
+
+    def depolarize1(self, q: int, p: float):
+        check(lib().ppvm_depolarize1(self.h, int(q), float(p)))
+
+    def depolarize2(self, a: int, b: int, p: float):
+        check(lib().ppvm_depolarize2(self.h, int(a), int(b), float(p)))
+
+    def two_qubit_pauli_error(self, a: int, b: int, p):
+        arr = np.ascontiguousarray(np.asarray(p, dtype=np.float64).reshape(15))
+        check(lib().ppvm_two_qubit_pauli_error(self.h, int(a), int(b), f64p(arr)))
+
+    def amplitude_damping(self, q: int, gamma: float):
+        check(lib().ppvm_amplitude_damping(self.h, int(q), float(gamma)))
+
+    def overlap(self, other: "Handle") -> float:
+        return overlap(self, other)
+
     def truncate(self):
         check(lib().ppvm_truncate(self.h))
 
@@ -246,3 +284,145 @@ def overlap(a: Handle, b: Handle) -> float:
     v = ctypes.c_double()
     check(lib().ppvm_overlap(a.h, b.h, ctypes.byref(v)))
     return v.value
+
+
+def _lcheck(rc: int) -> None:
+    check(rc, lambda: lib().ppvm_lossy_last_error())
+
+
+class LossyHandle:
+    """Owning wrapper of a ``ppvm_lossy_handle``: the same methods as
+    :class:`Handle` (gates become ``ppvm_op`` programs) plus the loss channels."""
+
+    def __init__(self, n_qubits: int, min_abs_coeff: float, max_pauli_weight: int | None,
+                 max_loss_weight: int | None = None, capacity_hint: int = 0, _raw=None):
+        L = lib()
+        if _raw is not None:
+            self.h = _raw
+        else:
+            h = _H()
+            mw = (1 << 64) - 1 if max_pauli_weight is None else int(max_pauli_weight)
+            mlw = (1 << 64) - 1 if max_loss_weight is None else int(max_loss_weight)
+            _lcheck(L.ppvm_lossy_create(int(n_qubits), float(min_abs_coeff), mw, mlw, int(capacity_hint),
+                                        ctypes.byref(h)))
+            self.h = h
+        w = ctypes.c_uint32()
+        _lcheck(L.ppvm_lossy_words(self.h, ctypes.byref(w)))
+        self.W = int(w.value)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.ppvm_lossy_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def clone(self) -> "LossyHandle":
+        h = _H()
+        _lcheck(lib().ppvm_lossy_clone(self.h, ctypes.byref(h)))
+        return LossyHandle(0, 0.0, None, _raw=h)
+
+    def _words(self, *planes):
+        return [np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, self.W) for a in planes]
+
+    def add_words(self, x, z, l, c):
+        x, z, l = self._words(x, z, l)
+        c = np.ascontiguousarray(c, dtype=np.float64).reshape(-1)
+        if not (len(x) == len(z) == len(l) == len(c)):
+            raise ValueError("Initial terms and coefficients need to be of same length!")
+        _lcheck(lib().ppvm_lossy_add_terms(self.h, u64p(x), u64p(z), u64p(l), f64p(c), len(c)))
+
+    def set_preserve(self, x, z, l):
+        x, z, l = self._words(x, z, l)
+        _lcheck(lib().ppvm_lossy_set_preserve(self.h, u64p(x), u64p(z), u64p(l), len(x)))
+
+    def apply(self, program: np.ndarray):
+        program = np.ascontiguousarray(program, dtype=OP_DTYPE)
+        _lcheck(lib().ppvm_lossy_apply(self.h, program.ctypes.data, len(program)))
+
+    def _run(self, rows):
+        prog = _ops.Program()
+        for r in rows:
+            prog.add(*r[:4], p=r[4] if len(r) > 4 else None)
+        self.apply(prog.array())
+
+    _CODES = {**_ops.CLIFFORD_1Q, **_ops.CLIFFORD_2Q, **_ops.ROTATION_1Q, **_ops.ROTATION_2Q}
+
+    def gate(self, name: str, targets, theta: float | None = None):
+        t = np.asarray(targets, dtype=np.int64).reshape(-1)
+        if (t < 0).any():
+            raise ValueError("negative qubit index")
+        code = self._CODES[name]
+        th = 0.0 if theta is None else float(theta)
+        if code in _ops.TWO_QUBIT:
+            if len(t) % 2:
+                raise ValueError("Expected an even number of targets for a two-qubit gate")
+            self._run([(code, int(a), int(b), th) for a, b in zip(t[0::2], t[1::2])])
+        else:
+            self._run([(code, int(q), 0, th) for q in t])
+
+    def r(self, q: int, axis_angle: float, theta: float):
+        """rz(phi); rx(theta); rz(-phi) (crates/ppvm-pauli-sum/src/sum/rot1.rs:135-143)."""
+        self._run([(_ops.RZ, q, 0, axis_angle), (_ops.RX, q, 0, theta), (_ops.RZ, q, 0, -axis_angle)])
+
+    def pauli_error(self, q: int, p):
+        self._run([(_ops.PAULI_ERROR, q, 0, 0.0, list(np.asarray(p, dtype=np.float64).reshape(3)))])
+
+    def depolarize1(self, q: int, p: float):
+        self._run([(_ops.DEPOLARIZE1, q, 0, 0.0, [p])])
+
+    def depolarize2(self, a: int, b: int, p: float):
+        self._run([(_ops.DEPOLARIZE2, a, b, 0.0, [p])])
+
+    def two_qubit_pauli_error(self, a: int, b: int, p):
+        self._run([(_ops.PAULI_ERROR2, a, b, 0.0, list(np.asarray(p, dtype=np.float64).reshape(15)))])
+
+    def amplitude_damping(self, q: int, gamma: float):
+        self._run([(_ops.AMPLITUDE_DAMPING, q, 0, 0.0, [gamma])])
+
+    def loss_channel(self, q: int, p: float):
+        self._run([(_ops.LOSS, q, 0, 0.0, [p])])
+
+    def correlated_loss_channel(self, a: int, b: int, p):
+        self._run([(_ops.CORRELATED_LOSS, a, b, 0.0, list(np.asarray(p, dtype=np.float64).reshape(3)))])
+
+    def reset_loss_channel(self, q: int):
+        self._run([(_ops.RESET_LOSS, q, 0, 0.0)])
+
+    def truncate(self):
+        self._run([(_ops.TRUNCATE, 0, 0, 0.0)])
+
+    def synchronize(self):
+        pass
+
+    def overlap_with_zero(self) -> float:
+        v = ctypes.c_double()
+        _lcheck(lib().ppvm_lossy_overlap_with_zero(self.h, ctypes.byref(v)))
+        return v.value
+
+    def trace(self, pattern: str) -> float:
+        v = ctypes.c_double()
+        _lcheck(lib().ppvm_lossy_trace(self.h, pattern.encode(), ctypes.byref(v)))
+        return v.value
+
+    def overlap(self, other: "LossyHandle") -> float:
+        v = ctypes.c_double()
+        _lcheck(lib().ppvm_lossy_overlap(self.h, other.h, ctypes.byref(v)))
+        return v.value
+
+    def __len__(self) -> int:
+        v = ctypes.c_uint64()
+        _lcheck(lib().ppvm_lossy_len(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    def export(self):
+        n = len(self)
+        x, z, l = (np.zeros((n, self.W), dtype=np.uint64) for _ in range(3))
+        c = np.zeros(n, dtype=np.float64)
+        got = ctypes.c_size_t()
+        _lcheck(lib().ppvm_lossy_export_terms(self.h, u64p(x), u64p(z), u64p(l), f64p(c), n, ctypes.byref(got)))
+        return x, z, l, c
+
+    def stats(self) -> dict:
+        return {"n_live": len(self)}

@@ -86,6 +86,10 @@ bool is_map_op(u32 c) {
 // ops after which a truncated state stays truncated (sign flips only)
 bool keeps_clean(u32 c) { return c == PPVM_OP_X || c == PPVM_OP_Y || c == PPVM_OP_Z; }
 
+}  // namespace
+
+namespace ppvm {
+
 // two_qubit_pauli_error: indices of the channel Paulis
 // (IX,IY,IZ,XI,XX,XY,XZ,YI,YX,YY,YZ,ZI,ZX,ZY,ZZ) anticommuting with (P_a,P_b),
 // in the summation order of crates/ppvm-pauli-sum/src/sum/noise.rs:56-106
@@ -102,7 +106,7 @@ const int kPe2Rows[16][8] = {
 // Pauli code x|z<<1 (I0 X1 Z2 Y3) -> letter index I0 X1 Y2 Z3 of the rows above.
 const int kCodeToLetter[4] = {0, 1, 3, 2};
 
-}  // namespace
+}  // namespace ppvm
 
 struct ppvm_sum {
   u32 n = 0, W = 1;  // W: words per plane inside the engine (1, 2, 4 or 8)
@@ -547,6 +551,10 @@ int compact_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_compact); }
 int overlap2_w(ppvm_sum* s, ppvm_sum* b) { PPVM_DISPATCH_W(s, launch_overlap2, b); }
 int trace_w(ppvm_sum* s, const PatternProg& prog) { PPVM_DISPATCH_W(s, launch_trace, prog); }
 
+}  // namespace
+
+namespace ppvm {
+
 // Pauli pattern text -> matcher items (crates/ppvm-pauli-word/src/pattern/parse.rs:30-143).
 // Symbol sets are kept as accepted Pauli codes; '?' adds the identity to one or
 // two letters, and turns the three-letter set into the "any Pauli or identity"
@@ -621,6 +629,10 @@ int compile_pattern(const char* text, u32 nq, PatternProg& prog) {
   }
   return PPVM_OK;
 }
+
+}  // namespace ppvm
+
+namespace {
 
 int launch_skip(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;

@@ -24,6 +24,7 @@ CNOT, CZ, CY = 11, 12, 13
 RX, RY, RZ, RXX, RYY, RZZ = 20, 21, 22, 23, 24, 25
 PAULI_ERROR, DEPOLARIZE1, DEPOLARIZE2, PAULI_ERROR2 = 40, 41, 42, 43
 AMPLITUDE_DAMPING = 44
+LOSS, CORRELATED_LOSS, RESET_LOSS = 45, 46, 47  # lossy sums only
 TRUNCATE = 60
 
 CLIFFORD_1Q = {
@@ -34,7 +35,8 @@ CLIFFORD_2Q = {"cnot": CNOT, "cz": CZ, "cy": CY}
 ROTATION_1Q = {"rx": RX, "ry": RY, "rz": RZ}
 ROTATION_2Q = {"rxx": RXX, "ryy": RYY, "rzz": RZZ}
 ROTATIONS = set(ROTATION_1Q.values()) | set(ROTATION_2Q.values())
-TWO_QUBIT = set(CLIFFORD_2Q.values()) | set(ROTATION_2Q.values()) | {DEPOLARIZE2, PAULI_ERROR2}
+TWO_QUBIT = (set(CLIFFORD_2Q.values()) | set(ROTATION_2Q.values())
+             | {DEPOLARIZE2, PAULI_ERROR2, CORRELATED_LOSS})
 
 OP_DTYPE = np.dtype(
     [
@@ -127,6 +129,48 @@ def unpack_words(x: np.ndarray, z: np.ndarray, n_qubits: int) -> list[str]:
             zb = (int(zs[t, w]) >> b) & 1
             chars.append("IXZY"[xb | (zb << 1)])
         out.append("".join(chars))
+    return out
+
+
+def pack_lossy_strings(strings, n_qubits: int):
+    """Pack lossy Pauli strings (I X Y Z L, '_' skipped; loss/data.rs:33-45)
+    into ``(x, z, l)`` uint64 arrays of shape (count, W)."""
+    W = words_for(n_qubits)
+    cnt = len(strings)
+    l = np.zeros((cnt, W), dtype=np.uint64)
+    plain = []
+    for t, s in enumerate(strings):
+        chars = []
+        q = 0
+        for ch in s:
+            if ch == "_":
+                continue
+            if ch == "L":
+                if q >= n_qubits:
+                    raise ValueError(f"Pauli string {s!r} longer than n_qubits={n_qubits}")
+                w, b = divmod(q, 64)
+                l[t, w] |= np.uint64(1) << np.uint64(b)
+                ch = "I"
+            chars.append(ch)
+            q += 1
+        plain.append("".join(chars))
+    x, z = pack_strings(plain, n_qubits)
+    return x, z, l
+
+
+def unpack_lossy_words(x: np.ndarray, z: np.ndarray, l: np.ndarray, n_qubits: int) -> list[str]:
+    """Inverse of :func:`pack_lossy_strings`."""
+    out = unpack_words(x, z, n_qubits)
+    ls = np.asarray(l, dtype=np.uint64)
+    for t in range(ls.shape[0]):
+        if not ls[t].any():
+            continue
+        chars = list(out[t])
+        for q in range(n_qubits):
+            w, b = divmod(q, 64)
+            if (int(ls[t, w]) >> b) & 1:
+                chars[q] = "L"
+        out[t] = "".join(chars)
     return out
 
 

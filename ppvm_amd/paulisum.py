@@ -18,7 +18,7 @@ from typing import Any
 import numpy as np
 
 from . import _native
-from .ops import pack_strings, unpack_words, weight_of
+from .ops import pack_lossy_strings, pack_strings, unpack_lossy_words, unpack_words, weight_of
 
 _COMPACT_RE = re.compile(r"^([IXYZ]\d+)+$")
 _COMPACT_TOKEN_RE = re.compile(r"([IXYZ])(\d+)")
@@ -109,8 +109,6 @@ class PauliSum:
                 raise ValueError(
                     "All terms must have the same length! Expected length "
                     f"{n_qubits}, but got term of length {len(term)}: {term!r}")
-        if max_loss_weight is not None:
-            raise NotImplementedError("lossy Pauli words are not supported by this engine")
         coefficients = tuple(float(c) for c in coefficients)
         if not coefficients:
             coefficients = (1.0,) * len(initial_terms)
@@ -123,16 +121,22 @@ class PauliSum:
         self.max_pauli_weight = max_pauli_weight
         self.max_loss_weight = max_loss_weight
         self.preserve_strings = tuple(preserve_strings) if preserve_strings else None
-        self._h = _native.Handle(self.n_qubits, self.min_abs_coeff, max_pauli_weight, capacity_hint)
+        self._h = self._new_handle(capacity_hint)
         if self.preserve_strings:
             for s in self.preserve_strings:
                 if len(s) != n_qubits:
                     raise ValueError("All preserve strings must have length n_qubits "
                                      f"({n_qubits}); got {len(s)}: {s!r}")
-            px, pz = pack_strings(self.preserve_strings, self.n_qubits)
-            self._h.set_preserve(px, pz)
-        x, z = pack_strings(self.initial_terms, self.n_qubits)
-        self._h.add_words(x, z, np.asarray(coefficients, dtype=np.float64))
+            self._h.set_preserve(*self._pack(self.preserve_strings))
+        self._h.add_words(*self._pack(self.initial_terms), np.asarray(coefficients, dtype=np.float64))
+
+    def _new_handle(self, capacity_hint: int):
+        # max_loss_weight only applies to lossy words; like the reference's
+        # non-lossy classes, this one accepts and ignores it (interface.rs:89)
+        return _native.Handle(self.n_qubits, self.min_abs_coeff, self.max_pauli_weight, capacity_hint)
+
+    def _pack(self, strings):
+        return pack_strings(strings, self.n_qubits)
 
     @classmethod
     def new(cls, n_qubits: int, terms, min_abs_coeff: float = 1e-10,
@@ -209,7 +213,9 @@ class PauliSum:
 
     def overlap(self, other: "PauliSum") -> float:
         """sum_k other[k] * self[k] (crates/ppvm-pauli-sum/src/sum/trace.rs:30-37)."""
-        return _native.overlap(self._h, other._h)
+        if type(other._h) is not type(self._h):
+            raise TypeError("overlap needs two sums of the same kind")
+        return self._h.overlap(other._h)
 
     def truncate(self) -> None:
         self._h.truncate()
@@ -271,16 +277,15 @@ class PauliSum:
 
     def r(self, addr0: int, axis_angle: float, theta: float, *, truncate: bool = True):
         """R(phi, theta) = rz(phi); rx(theta); rz(-phi) (crates/ppvm-pauli-sum/src/sum/rot1.rs:135-143)."""
-        _native.check(_native.lib().ppvm_r(self._h.h, int(addr0), float(axis_angle), float(theta)))
+        self._h.r(int(addr0), float(axis_angle), float(theta))
         if truncate:
             self._h.truncate()
 
     # Noise (TruncatingNoiseMixin)
     def pauli_error(self, *args, p=None, truncate: bool = True):
         t, pv, tr = _split(args, p, "p", truncate)
-        arr = np.ascontiguousarray(np.asarray(pv, dtype=np.float64).reshape(3))
         for q in t:
-            _native.check(_native.lib().ppvm_pauli_error(self._h.h, int(q), _native.f64p(arr)))
+            self._h.pauli_error(int(q), pv)
         if tr:
             self._h.truncate()
 
@@ -299,7 +304,7 @@ class PauliSum:
     def depolarize1(self, *args, p=None, truncate: bool = True):
         t, pv, tr = _split(args, p, "p", truncate)
         for q in t:
-            _native.check(_native.lib().ppvm_depolarize1(self._h.h, int(q), float(pv)))
+            self._h.depolarize1(int(q), float(pv))
         if tr:
             self._h.truncate()
 
@@ -307,7 +312,7 @@ class PauliSum:
         """Amplitude damping (ppvm-python/src/ppvm/paulisum.py:389-403): X and Y on
         the qubit are damped by sqrt(1 - gamma); Z branches into
         gamma * I + (1 - gamma) * Z."""
-        _native.check(_native.lib().ppvm_amplitude_damping(self._h.h, int(addr0), float(gamma)))
+        self._h.amplitude_damping(int(addr0), float(gamma))
         if truncate:
             self._h.truncate()
 
@@ -316,7 +321,7 @@ class PauliSum:
         if len(t) % 2:
             raise ValueError("Expected an even number of targets for a two-qubit channel")
         for a, b in zip(t[0::2], t[1::2]):
-            _native.check(_native.lib().ppvm_depolarize2(self._h.h, int(a), int(b), float(pv)))
+            self._h.depolarize2(int(a), int(b), float(pv))
         if tr:
             self._h.truncate()
 
@@ -328,10 +333,8 @@ class PauliSum:
         t, pv, tr = _split(args, p, "p", truncate)
         if len(t) % 2:
             raise ValueError("Expected an even number of targets for a two-qubit channel")
-        arr = np.ascontiguousarray(np.asarray(pv, dtype=np.float64).reshape(15))
         for a, b in zip(t[0::2], t[1::2]):
-            _native.check(_native.lib().ppvm_two_qubit_pauli_error(self._h.h, int(a), int(b),
-                                                                    _native.f64p(arr)))
+            self._h.two_qubit_pauli_error(int(a), int(b), pv)
         if tr:
             self._h.truncate()
 
@@ -341,3 +344,64 @@ class PauliSum:
         arr = program.array() if hasattr(program, "array") else program
         self._h.apply(arr)
 
+
+class LossyPauliSum(PauliSum):
+    """A PauliSum over lossy words: the Pauli alphabet {I, X, Y, Z, L}, where
+    ``L`` marks a lost qubit (ppvm-python/src/ppvm/paulisum.py:420-520,
+    crates/ppvm-pauli-word/src/loss/data.rs).  Gates skip lost qubits as the
+    reference does; ``max_loss_weight`` drops terms with more lost qubits than
+    that; the loss channels below are available on this class only.
+    """
+
+    def _new_handle(self, capacity_hint: int):
+        return _native.LossyHandle(self.n_qubits, self.min_abs_coeff, self.max_pauli_weight,
+                                   self.max_loss_weight, capacity_hint)
+
+    def _pack(self, strings):
+        return pack_lossy_strings(strings, self.n_qubits)
+
+    @property
+    def terms(self) -> list[tuple[str, float]]:
+        x, z, l, c = self._export()
+        return list(zip(unpack_lossy_words(x, z, l, self.n_qubits), c.tolist()))
+
+    def weights(self) -> list[tuple[str, int]]:
+        x, z, l, _ = self._export()
+        return list(zip(unpack_lossy_words(x, z, l, self.n_qubits),
+                        weight_of(np.bitwise_or(x, l), z).tolist()))
+
+    def current_max_weight(self) -> int:
+        x, z, l, _ = self._export()
+        return int(weight_of(np.bitwise_or(x, l), z).max()) if len(x) else 0
+
+    def loss_weights(self) -> list[tuple[str, int]]:
+        x, z, l, _ = self._export()
+        return list(zip(unpack_lossy_words(x, z, l, self.n_qubits), weight_of(l, l).tolist()))
+
+    def trace(self, pattern: str) -> float:
+        return self._h.trace(pattern)
+
+    def loss_channel(self, addr0: int, p: float, *, truncate: bool = True) -> None:
+        """Single-qubit loss (crates/ppvm-pauli-sum/src/sum/noise.rs:159-180): a
+        term with L on the qubit adds p * (word with I there); any other term is
+        scaled by (1 - p)."""
+        self._h.loss_channel(int(addr0), float(p))
+        if truncate:
+            self._h.truncate()
+
+    def correlated_loss_channel(self, addr0: int, addr1: int, p: Sequence[float], *,
+                                truncate: bool = True) -> None:
+        """Correlated loss of two qubits (noise.rs:182-248); ``p`` = (both lost,
+        one of two lost, the second lost once the first is)."""
+        if len(p) != 3:
+            raise ValueError("correlated_loss_channel takes three probabilities")
+        self._h.correlated_loss_channel(int(addr0), int(addr1), [float(v) for v in p])
+        if truncate:
+            self._h.truncate()
+
+    def reset_loss_channel(self, addr0: int, *, truncate: bool = True) -> None:
+        """Reset a lost qubit to |0> (noise.rs:250-274): I and Z terms add the
+        word with L on the qubit; L terms are zeroed."""
+        self._h.reset_loss_channel(int(addr0))
+        if truncate:
+            self._h.truncate()

@@ -1,0 +1,185 @@
+"""Lossy sums on the GPU (ppvm_lossy_* through the C ABI) against the CPU
+oracle: identical words, order and coefficient bits after every segment of
+random circuits that mix Cliffords, rotations, noise, loss, correlated loss,
+reset-loss and truncation; the reference's known answers through the Python
+``LossyPauliSum`` (ppvm-python/test/test_loss.py, crates/ppvm-pauli-sum/tests/loss.rs);
+preserve sets, loss-weight caps, patterns, overlaps and error paths.
+"""
+
+import numpy as np
+import pytest
+
+from oracle.pyoracle import LossyOracleSum
+from ppvm_amd import ops as O
+
+pytestmark = pytest.mark.gpu
+
+KINDS = [O.X, O.Y, O.Z, O.H, O.S, O.S_DAG, O.SQRT_X, O.SQRT_X_DAG, O.SQRT_Y, O.SQRT_Y_DAG, O.CNOT, O.CZ,
+         O.CY, O.RX, O.RY, O.RZ, O.RXX, O.RYY, O.RZZ, O.PAULI_ERROR, O.DEPOLARIZE1, O.DEPOLARIZE2,
+         O.PAULI_ERROR2, O.AMPLITUDE_DAMPING, O.LOSS, O.LOSS, O.RESET_LOSS, O.RESET_LOSS,
+         O.CORRELATED_LOSS, O.CORRELATED_LOSS]
+
+
+def _handle(n, tau=0.0, maxw=None, maxlw=None):
+    from ppvm_amd._native import LossyHandle
+    return LossyHandle(n, tau, maxw, maxlw)
+
+
+def _words(rng, n, count, support):
+    out = set()
+    while len(out) < count:
+        chars = ["I"] * n
+        for q in rng.choice(support, min(len(support), int(rng.integers(1, 5))), replace=False):
+            chars[int(q)] = "IXYZL"[int(rng.integers(0, 5))]
+        out.add("".join(chars))
+    return sorted(out)
+
+
+def _program(rng, support, count, trunc_p=0.5):
+    p = O.Program()
+    for _ in range(count):
+        k = int(rng.choice(KINDS))
+        a, b = (int(v) for v in rng.choice(support, 2, replace=False))
+        p.add(k, a, b, float(rng.uniform(-3, 3)), list(rng.uniform(0.0, 0.08, 15)))
+        if rng.random() < trunc_p:
+            p.truncate()
+    return p.array()
+
+
+def _same(g, o, ctx):
+    gx, gz, gl, gc = g.export()
+    ox, oz, ol, oc = o.export()
+    assert len(gc) == len(oc), f"{ctx}: {len(gc)} GPU terms vs {len(oc)} oracle terms"
+    assert np.array_equal(gx, ox) and np.array_equal(gz, oz) and np.array_equal(gl, ol), \
+        f"{ctx}: words or order differ"
+    bad = np.nonzero(gc.view(np.uint64) != oc.view(np.uint64))[0]
+    assert len(bad) == 0, f"{ctx}: {len(bad)} coefficients differ, first {gc[bad[0]]!r} vs {oc[bad[0]]!r}"
+
+
+def _load(g, o, words, coefs):
+    from ppvm_amd.ops import pack_lossy_strings
+    x, z, l = pack_lossy_strings(words, o.n_qubits)
+    g.add_words(x, z, l, np.asarray(coefs))
+    o.add_lossy_words(x, z, l, np.asarray(coefs))
+
+
+@pytest.mark.parametrize("n,seed,tau,maxw,maxlw", [
+    (6, 0, 0.0, None, None), (9, 1, 1e-4, None, 2), (12, 2, 1e-6, 7, None),
+    (70, 3, 1e-5, 8, 3), (130, 4, 1e-5, None, 2)])
+def test_random_lossy_circuits(n, seed, tau, maxw, maxlw):
+    rng = np.random.default_rng(seed)
+    support = np.array(sorted(rng.choice(n, min(n, 8), replace=False)))
+    print(f"lossy n={n} seed={seed}", flush=True)
+    g, o = _handle(n, tau, maxw, maxlw), LossyOracleSum(n, tau, maxw, maxlw)
+    words = _words(rng, n, 24, support)
+    _load(g, o, words, rng.uniform(-1, 1, len(words)))
+    prog = _program(rng, support, 160)
+    for s in range(0, len(prog), 20):
+        seg = prog[s:s + 20]
+        g.apply(seg)
+        o.apply(seg)
+        _same(g, o, f"n={n} seed={seed} ops<{s + 20}")
+        assert abs(g.overlap_with_zero() - o.overlap_with_zero()) <= 1e-10
+    for pat in ("Z?*", "[XZ]?*", "Z0", "X?{3}Z?*"):
+        assert abs(g.trace(pat) - o.trace(pat)) <= 1e-10, pat
+
+
+def test_many_terms_grow_and_merge():
+    # tens of thousands of words: multi-block scans, arena growth during a
+    # branch op, and every reset / correlated child landing on existing words
+    rng = np.random.default_rng(7)
+    n = 16
+    support = np.arange(n)
+    g, o = _handle(n, 1e-9), LossyOracleSum(n, 1e-9)
+    words = sorted({"".join(rng.choice(list("IXYZL"), n, p=[0.3, 0.15, 0.15, 0.25, 0.15])) for _ in range(8000)})
+    _load(g, o, words, rng.uniform(-1, 1, len(words)))
+    prog = O.Program()
+    for q in range(0, 4):
+        prog.add(O.RESET_LOSS, q).add(O.CORRELATED_LOSS, q, q + 8, p=[0.02, 0.03, 0.05]).add(O.LOSS, q + 8, p=[0.1])
+        prog.add(O.RZZ, q, q + 1, theta=0.3).add(O.RX, q + 8, theta=0.2).truncate()
+    arr = prog.array()
+    for s in range(0, len(arr), 6):
+        g.apply(arr[s:s + 6])
+        o.apply(arr[s:s + 6])
+        _same(g, o, f"grow ops<{s + 6}")
+    assert len(o) > 2 * len(words)
+
+
+def test_preserve_and_overlap_and_clone():
+    n = 10
+    rng = np.random.default_rng(11)
+    support = np.arange(n)
+    from ppvm_amd.ops import pack_lossy_strings
+    keep = ["LZIIIIIIII", "XXIIIIIIIL"]
+    a, oa = _handle(n, 0.05), LossyOracleSum(n, 0.05)
+    for h in (a,):
+        h.set_preserve(*pack_lossy_strings(keep, n))
+    oa.set_preserve(keep)
+    words = _words(rng, n, 40, support) + keep
+    coefs = list(rng.uniform(-1, 1, len(words) - 2)) + [0.01, -0.02]
+    _load(a, oa, words, coefs)
+    prog = _program(rng, support, 60, trunc_p=0.7)
+    a.apply(prog)
+    oa.apply(prog)
+    _same(a, oa, "preserve")
+    b, ob = a.clone(), oa.copy()
+    prog2 = _program(np.random.default_rng(12), support, 30)
+    b.apply(prog2)
+    ob.apply(prog2)
+    _same(b, ob, "clone")
+    _same(a, oa, "original after clone")
+    assert abs(a.overlap(b) - oa.overlap(ob)) <= 1e-10
+
+
+def test_python_lossy_known_answers():
+    from ppvm_amd import LossyPauliSum
+    p = 0.1
+    ps = LossyPauliSum.new(n_qubits=2, terms=["ZZ"])
+    ps.reset_loss_channel(0)
+    ps.loss_channel(0, p)
+    ps.cnot(0, 1)
+    ps.h(0)
+    assert abs(ps.overlap_with_zero() - (1 - p)) < 1e-9
+    for obs, expect in (("ZZ", 1.0), ("XX", 1 - 0.1)):
+        ps = LossyPauliSum.new(n_qubits=2, terms=[obs])
+        ps.reset_loss_channel(0)
+        ps.reset_loss_channel(1)
+        ps.correlated_loss_channel(0, 1, [0.1, 0.0, 0.0])
+        ps.cnot(0, 1)
+        ps.h(0)
+        assert abs(ps.overlap_with_zero() - expect) < 1e-9
+    ps = LossyPauliSum.new(n_qubits=1, terms=[("I", 2.0), ("Z", 3.0)])
+    ps.reset_loss_channel(0)
+    assert ps.terms == [("I", 2.0), ("Z", 3.0), ("L", 5.0)]
+    assert dict(ps.weights()) == {"I": 0, "Z": 1, "L": 1}
+    ps = LossyPauliSum.new(n_qubits=3, terms=["ZZZ"], max_loss_weight=2)
+    for q in range(3):
+        ps.reset_loss_channel(q, truncate=False)
+    for q in range(3):
+        ps.loss_channel(q, 0.1, truncate=False)
+    before = len(ps)
+    ps.truncate()
+    assert len(ps) == before - 1
+    assert max(w for _, w in ps.loss_weights()) == 2
+
+
+def test_errors():
+    from ppvm_amd import LossyPauliSum, PauliSum
+    from ppvm_amd.ops import pack_lossy_strings
+    g = _handle(4)
+    x, z, l = pack_lossy_strings(["XIII"], 4)
+    with pytest.raises(ValueError):
+        g.add_words(x, z, x, np.ones(1))  # X and L on one qubit
+    with pytest.raises(ValueError):
+        g.apply(O.Program().add(O.LOSS, 4, p=[0.1]).array())
+    with pytest.raises(ValueError):
+        g.apply(O.Program().add(O.CORRELATED_LOSS, 1, 1, p=[0.1, 0, 0]).array())
+    with pytest.raises(ValueError):
+        g.trace("Q0")
+    plain = PauliSum.new(4, "ZIII")
+    with pytest.raises(ValueError):
+        plain.apply(O.Program().add(O.LOSS, 0, p=[0.1]).array())
+    with pytest.raises(ValueError):
+        LossyPauliSum.new(2, "ZQ")
+    with pytest.raises(TypeError):
+        LossyPauliSum.new(2, "ZZ").overlap(PauliSum.new(2, "ZZ"))
