@@ -532,7 +532,8 @@ u32 lgrid(u64 n) { return (u32)std::max<u64>(1, std::min<u64>((n + kLB - 1) / kL
 }  // namespace
 
 struct ppvm_lossy_sum {
-  u32 n = 0, W = 1;
+  u32 n = 0, W = 1;  // W: words per plane on the device (1, 2, 4 or 8)
+  u32 Wa = 1;        // words per plane at the ABI: ceil(n / 64)
   double tau = 0.0;
   u64 maxw = ~0ull, maxlw = ~0ull;
   int device = 0;
@@ -1092,15 +1093,15 @@ struct LKeyHash {
   }
 };
 
-// Words must satisfy x = z = 0 wherever l = 1 and have no bits at or above n.
+// Words (Wa words per plane) must satisfy x = z = 0 wherever l = 1 and have no
+// bits at or above n.
 int check_words(const ppvm_lossy_sum* s, const uint64_t* x, const uint64_t* z, const uint64_t* l, size_t count) {
-  const u32 W = s->W, Wa = (s->n + 63) / 64;
+  const u32 Wa = s->Wa;
   for (size_t i = 0; i < count; ++i)
-    for (u32 p = 0; p < W; ++p) {
-      const u64 xv = x[i * W + p], zv = z[i * W + p], lv = l[i * W + p];
+    for (u32 p = 0; p < Wa; ++p) {
+      const u64 xv = x[i * Wa + p], zv = z[i * Wa + p], lv = l[i * Wa + p];
       u64 valid = ~0ull;
-      if (p >= Wa) valid = 0;
-      else if (p == Wa - 1 && (s->n & 63)) valid = (1ull << (s->n & 63)) - 1;
+      if (p == Wa - 1 && (s->n & 63)) valid = (1ull << (s->n & 63)) - 1;
       if (((xv | zv | lv) & ~valid) != 0)
         return lfail(PPVM_EINVAL, "term %zu has bits beyond qubit %u", i, s->n - 1);
       if (lv & (xv | zv)) return lfail(PPVM_EINVAL, "term %zu: a lost qubit cannot also carry X, Y or Z", i);
@@ -1131,6 +1132,7 @@ int ppvm_lossy_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_paul
   ppvm_lossy_sum* s = new ppvm_lossy_sum();
   s->n = n_qubits;
   s->W = W;
+  s->Wa = Wa;
   s->tau = min_abs_coeff;
   s->maxw = max_pauli_weight;
   s->maxlw = max_loss_weight;
@@ -1168,7 +1170,7 @@ int ppvm_lossy_destroy(ppvm_lossy_handle s) {
 
 int ppvm_lossy_words(ppvm_lossy_handle s, uint32_t* w) {
   if (!s || !w) return lfail(PPVM_EINVAL, "null argument");
-  *w = s->W;
+  *w = s->Wa;
   return PPVM_OK;
 }
 
@@ -1209,13 +1211,13 @@ int ppvm_lossy_set_preserve(ppvm_lossy_handle s, const uint64_t* x, const uint64
   if (count && (!x || !z || !l)) return lfail(PPVM_EINVAL, "null argument");
   if (count > (1u << 20)) return lfail(PPVM_EINVAL, "preserve set of %zu words is too large", count);
   if ((rc = check_words(s, x, z, l, count))) return rc;
-  const u32 W = s->W;
-  std::vector<u64> keys((size_t)count * 3 * W);
+  const u32 W = s->W, Wa = s->Wa;
+  std::vector<u64> keys((size_t)count * 3 * W, 0ull);
   for (size_t i = 0; i < count; ++i)
-    for (u32 p = 0; p < W; ++p) {
-      keys[i * 3 * W + p] = x[i * W + p];
-      keys[i * 3 * W + W + p] = z[i * W + p];
-      keys[i * 3 * W + 2 * W + p] = l[i * W + p];
+    for (u32 p = 0; p < Wa; ++p) {
+      keys[i * 3 * W + p] = x[i * Wa + p];
+      keys[i * 3 * W + W + p] = z[i * Wa + p];
+      keys[i * 3 * W + 2 * W + p] = l[i * Wa + p];
     }
   LHIPCHK(hipStreamSynchronize(s->stream));
   if (s->preserve) (void)hipFree(s->preserve);
@@ -1241,16 +1243,16 @@ int ppvm_lossy_add_terms(ppvm_lossy_handle s, const uint64_t* x, const uint64_t*
   std::vector<u64> keys;
   std::vector<double> cs;
   if ((rc = lexport(s, keys, cs))) return broken(s, rc);
-  const u32 W = s->W, np = 3 * W;
+  const u32 W = s->W, Wa = s->Wa, np = 3 * W;
   std::unordered_map<std::vector<u64>, size_t, LKeyHash> idx;
   for (size_t i = 0; i < cs.size(); ++i)
     idx.emplace(std::vector<u64>(keys.begin() + i * np, keys.begin() + (i + 1) * np), i);
   for (size_t i = 0; i < count; ++i) {
-    std::vector<u64> k(np);
-    for (u32 p = 0; p < W; ++p) {
-      k[p] = x[i * W + p];
-      k[W + p] = z[i * W + p];
-      k[2 * W + p] = l[i * W + p];
+    std::vector<u64> k(np, 0ull);
+    for (u32 p = 0; p < Wa; ++p) {
+      k[p] = x[i * Wa + p];
+      k[W + p] = z[i * Wa + p];
+      k[2 * W + p] = l[i * Wa + p];
     }
     auto it = idx.find(k);
     if (it != idx.end()) {
@@ -1305,12 +1307,12 @@ int ppvm_lossy_export_terms(ppvm_lossy_handle s, uint64_t* x, uint64_t* z, uint6
   std::vector<u64> keys;
   std::vector<double> cs;
   if ((rc = lexport(s, keys, cs))) return broken(s, rc);
-  const u32 W = s->W, np = 3 * W;
+  const u32 W = s->W, Wa = s->Wa, np = 3 * W;
   for (size_t i = 0; i < cs.size(); ++i) {
-    for (u32 p = 0; p < W; ++p) {
-      x[i * W + p] = keys[i * np + p];
-      z[i * W + p] = keys[i * np + W + p];
-      l[i * W + p] = keys[i * np + 2 * W + p];
+    for (u32 p = 0; p < Wa; ++p) {
+      x[i * Wa + p] = keys[i * np + p];
+      z[i * Wa + p] = keys[i * np + W + p];
+      l[i * Wa + p] = keys[i * np + 2 * W + p];
     }
     c[i] = cs[i];
   }
