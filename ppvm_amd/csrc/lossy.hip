@@ -75,7 +75,7 @@ int lfail(int code, const char* fmt, ...) {
 constexpr int kLB = 256;           // threads per block
 constexpr u32 kLGrid = 2048;       // grid of the grid-stride kernels
 constexpr u32 kLRedGrid = 512;     // blocks of the coefficient reductions
-constexpr u64 kLScanChunk = 4096;  // items per block of the prefix scans
+constexpr u64 kLScanChunk = 1024;  // items per block of the chunked passes and prefix scans
 constexpr u32 kLScanMaxBlocks = 65536;
 constexpr u64 kLMinCap = 1024;
 constexpr u64 kLMaxCap = 1ull << 30;
@@ -197,6 +197,21 @@ __global__ void k_lz_fill(u32* __restrict__ p, u64 n) {
   for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) p[i] = 0u;
 }
 
+// Sum of v over the block, written by thread 0 to *dst.
+__device__ __forceinline__ void block_total(u64 v, u64* dst) {
+  __shared__ u64 red[kLB / 64];
+#pragma unroll
+  for (u32 d = 32; d > 0; d >>= 1) v += __shfl_down(v, d, 64);
+  if ((threadIdx.x & 63u) == 0u) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    u64 t = 0;
+#pragma unroll
+    for (u32 w = 0; w < kLB / 64; ++w) t += red[w];
+    *dst = t;
+  }
+}
+
 // Words of the map are distinct, so an insert never compares keys.  The index
 // has at least 2n slots, so every probe ends.
 template <int W>
@@ -212,10 +227,15 @@ __global__ __launch_bounds__(kLB) void k_lz_index(const u64* __restrict__ planes
 
 // Parent pass: coefficient factor, local state change (map ops), child count
 // (branch ops).  cin may equal cout.
+// Block b covers [b * chunk, (b + 1) * chunk); for branch ops it also writes
+// the block's child total to bsum[b] (the first pass of the offset scan).
 template <int W>
-__global__ __launch_bounds__(kLB) void k_lz_apply(u64* planes, u64 cap, u64 n, const double* cin, double* cout,
-                                                  LTable tb, u32* __restrict__ cnt) {
-  for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) {
+__global__ __launch_bounds__(kLB) void k_lz_apply(u64* planes, u64 cap, u64 n, u64 chunk, const double* cin,
+                                                  double* cout, LTable tb, u32* __restrict__ cnt,
+                                                  u64* __restrict__ bsum) {
+  const u64 lo = (u64)blockIdx.x * chunk, hi = min(n, lo + chunk);
+  u64 acc = 0;
+  for (u64 i = lo + threadIdx.x; i < hi; i += kLB) {
     u64 k[3 * W];
     lload<W>(planes, cap, i, k);
     const u32 t = local_of<W>(k, tb);
@@ -223,11 +243,13 @@ __global__ __launch_bounds__(kLB) void k_lz_apply(u64* planes, u64 cap, u64 n, c
     cout[i] = __dmul_rn(cin[i], e.pf);
     if (tb.branch) {
       cnt[i] = e.nch;
+      acc += e.nch;
     } else if (tb.moves && e.pst != t) {
       put_local<W>(k, tb, e.pst);
       lstore<W>(planes, cap, i, k);
     }
   }
+  if (tb.branch) block_total(acc, &bsum[blockIdx.x]);
 }
 
 struct Contrib {
@@ -344,12 +366,17 @@ __global__ __launch_bounds__(kLB) void k_lz_merge(const u64* __restrict__ planes
 
 // Truncation flags: 1 = kept, 1 << 32 = dropped but preserved (restored at the
 // end in map order, sum/data.rs:296-310), 0 = dropped.
+// Chunked like k_lz_apply; bsum[b] = the block's packed flag total, also
+// added to *total (zeroed by the host before the launch).
 template <int W>
-__global__ __launch_bounds__(kLB) void k_lz_keep(const u64* __restrict__ planes, u64 cap, u64 n,
+__global__ __launch_bounds__(kLB) void k_lz_keep(const u64* __restrict__ planes, u64 cap, u64 n, u64 chunk,
                                                  const double* __restrict__ coef, double tau, u64 maxw, u64 maxlw,
                                                  const u64* __restrict__ preserve, u32 n_preserve,
-                                                 u64* __restrict__ flags) {
-  for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) {
+                                                 u64* __restrict__ flags, u64* __restrict__ bsum,
+                                                 u64* __restrict__ total) {
+  const u64 lo = (u64)blockIdx.x * chunk, hi = min(n, lo + chunk);
+  u64 acc = 0;
+  for (u64 i = lo + threadIdx.x; i < hi; i += kLB) {
     u64 k[3 * W];
     lload<W>(planes, cap, i, k);
     const double c = coef[i];
@@ -379,7 +406,10 @@ __global__ __launch_bounds__(kLB) void k_lz_keep(const u64* __restrict__ planes,
       }
     }
     flags[i] = f;
+    acc += f;
   }
+  block_total(acc, &bsum[blockIdx.x]);
+  if (threadIdx.x == 0) atomicAdd((unsigned long long*)total, (unsigned long long)bsum[blockIdx.x]);
 }
 
 template <int W>
@@ -497,27 +527,67 @@ __global__ __launch_bounds__(1024) void k_lz_bscan(u64* __restrict__ bsum, u32 G
   if (threadIdx.x == 0) *total = carry;
 }
 
+// Block b scans its chunk [b * chunk, ...) starting from the exclusive block
+// offset bsum[b]: tiles of kPer items per thread are staged through LDS (one
+// pad slot per kPer so the per-thread runs hit distinct banks), summed per
+// thread, combined across the wave with shuffles and across the four waves
+// through LDS, and written back coalesced.
+constexpr u32 kPer = 4;
+constexpr u32 kTile = kLB * kPer;
+static_assert(kLScanChunk % kTile == 0, "chunks are whole scan tiles");
+__device__ __forceinline__ u32 tpos(u32 e) { return e + e / kPer; }
+
 template <typename T>
 __global__ __launch_bounds__(kLB) void k_lz_scan(const T* __restrict__ in, u64 n, u64 chunk,
                                                  const u64* __restrict__ bsum, u64* __restrict__ out) {
-  __shared__ u64 s[kLB];
+  __shared__ u64 tile[kTile + kTile / kPer];
+  __shared__ u64 wsum[kLB / 64];
+  const u32 t = threadIdx.x, lane = t & 63u, wave = t >> 6;
   const u64 lo = (u64)blockIdx.x * chunk, hi = min(n, lo + chunk);
   u64 carry = bsum[blockIdx.x];
-  for (u64 base = lo; base < hi; base += kLB) {
-    const u64 i = base + threadIdx.x;
-    const u64 v = i < hi ? (u64)in[i] : 0ull;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < kLB; d <<= 1) {
-      const u64 add = (int)threadIdx.x >= d ? s[threadIdx.x - d] : 0ull;
-      __syncthreads();
-      s[threadIdx.x] += add;
-      __syncthreads();
+  for (u64 base = lo; base < hi; base += kTile) {
+#pragma unroll
+    for (u32 j = 0; j < kPer; ++j) {
+      const u32 e = j * kLB + t;
+      const u64 i = base + e;
+      tile[tpos(e)] = i < hi ? (u64)in[i] : 0ull;
     }
-    if (i < hi) out[i] = carry + s[threadIdx.x] - v;
-    const u64 tile = s[kLB - 1];
     __syncthreads();
-    carry += tile;
+    u64 loc[kPer];
+    u64 sum = 0;
+#pragma unroll
+    for (u32 j = 0; j < kPer; ++j) {
+      const u64 v = tile[tpos(t * kPer + j)];
+      loc[j] = sum;
+      sum += v;
+    }
+    u64 incl = sum;
+#pragma unroll
+    for (u32 d = 1; d < 64; d <<= 1) {
+      const u64 y = __shfl_up(incl, d, 64);
+      if (lane >= d) incl += y;
+    }
+    if (lane == 63u) wsum[wave] = incl;
+    __syncthreads();
+    u64 woff = 0, ttot = 0;
+#pragma unroll
+    for (u32 w = 0; w < kLB / 64; ++w) {
+      const u64 v = wsum[w];
+      woff += w < wave ? v : 0ull;
+      ttot += v;
+    }
+    const u64 off = carry + woff + (incl - sum);
+#pragma unroll
+    for (u32 j = 0; j < kPer; ++j) tile[tpos(t * kPer + j)] = off + loc[j];
+    __syncthreads();
+#pragma unroll
+    for (u32 j = 0; j < kPer; ++j) {
+      const u32 e = j * kLB + t;
+      const u64 i = base + e;
+      if (i < hi) out[i] = tile[tpos(e)];
+    }
+    carry += ttot;
+    __syncthreads();
   }
 }
 
@@ -622,18 +692,17 @@ int lgrow(ppvm_lossy_sum* s, u64 want) {
   return PPVM_OK;
 }
 
-template <typename T>
-int lscan(ppvm_lossy_sum* s, const T* in, u64 n, u64* out, u64* total) {
-  if (n == 0) {
-    *total = 0;
-    return PPVM_OK;
-  }
+// Items per block of the chunked passes: 4096, doubled until the blocks fit bsum.
+u64 scan_chunk(u64 n) {
   u64 chunk = kLScanChunk;
   while ((n + chunk - 1) / chunk > kLScanMaxBlocks) chunk *= 2;
-  const u32 G = (u32)((n + chunk - 1) / chunk);
-  hipLaunchKernelGGL(k_lz_bsum<T>, dim3(G), dim3(kLB), 0, s->stream, in, n, chunk, s->bsum);
+  return chunk;
+}
+u32 scan_blocks(u64 n, u64 chunk) { return (u32)std::max<u64>(1, (n + chunk - 1) / chunk); }
+
+// Block sums (already in s->bsum) -> exclusive block offsets; returns the total.
+int lscan_offsets(ppvm_lossy_sum* s, u32 G, u64* total) {
   hipLaunchKernelGGL(k_lz_bscan, dim3(1), dim3(1024), 0, s->stream, s->bsum, G, s->dtot);
-  hipLaunchKernelGGL(k_lz_scan<T>, dim3(G), dim3(kLB), 0, s->stream, in, n, chunk, s->bsum, out);
   LHIPCHK(hipGetLastError());
   LHIPCHK(hipMemcpyAsync(s->htot, s->dtot, 8, hipMemcpyDeviceToHost, s->stream));
   LHIPCHK(hipStreamSynchronize(s->stream));
@@ -641,12 +710,30 @@ int lscan(ppvm_lossy_sum* s, const T* in, u64 n, u64* out, u64* total) {
   return PPVM_OK;
 }
 
-int lindex(ppvm_lossy_sum* s, const u64* planes, u64 cap, u64 n, u32* table, u64 tcap);
+template <typename T>
+int lscan(ppvm_lossy_sum* s, const T* in, u64 n, u64* out, u64* total) {
+  if (n == 0) {
+    *total = 0;
+    return PPVM_OK;
+  }
+  const u64 chunk = scan_chunk(n);
+  const u32 G = scan_blocks(n, chunk);
+  hipLaunchKernelGGL(k_lz_bsum<T>, dim3(G), dim3(kLB), 0, s->stream, in, n, chunk, s->bsum);
+  int rc;
+  if ((rc = lscan_offsets(s, G, total))) return rc;
+  hipLaunchKernelGGL(k_lz_scan<T>, dim3(G), dim3(kLB), 0, s->stream, in, n, chunk, s->bsum, out);
+  LHIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
 
+// Index of n words in the first pow2(2n) slots of table (tcap slots); returns
+// the slot mask through *tmask.
 template <int W>
-int lindex_w(ppvm_lossy_sum* s, const u64* planes, u64 cap, u64 n, u32* table, u64 tcap) {
-  hipLaunchKernelGGL(k_lz_fill, dim3(lgrid(tcap)), dim3(kLB), 0, s->stream, table, tcap);
-  if (n) hipLaunchKernelGGL(k_lz_index<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, planes, cap, n, table, tcap - 1);
+int lindex_w(ppvm_lossy_sum* s, const u64* planes, u64 cap, u64 n, u32* table, u64 tcap, u64* tmask) {
+  const u64 slots = std::min(tcap, lpow2(std::max<u64>(2 * n, 64)));
+  *tmask = slots - 1;
+  hipLaunchKernelGGL(k_lz_fill, dim3(lgrid(slots)), dim3(kLB), 0, s->stream, table, slots);
+  if (n) hipLaunchKernelGGL(k_lz_index<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, planes, cap, n, table, slots - 1);
   LHIPCHK(hipGetLastError());
   return PPVM_OK;
 }
@@ -660,10 +747,6 @@ int lindex_w(ppvm_lossy_sum* s, const u64* planes, u64 cap, u64 n, u32* table, u
     default: return lfail(PPVM_EINVAL, "unsupported word count %u", W); \
   }
 
-int lindex(ppvm_lossy_sum* s, const u64* planes, u64 cap, u64 n, u32* table, u64 tcap) {
-  LDISPATCH(s->W, lindex_w, s, planes, cap, n, table, tcap);
-}
-
 // One op given its rule table.
 template <int W>
 int run_op(ppvm_lossy_sum* s, const LTable& tb) {
@@ -671,18 +754,21 @@ int run_op(ppvm_lossy_sum* s, const LTable& tb) {
   if (n == 0) return PPVM_OK;
   u64* P = s->planes[s->cur];
   const double* cin = s->coef[s->cur];
+  const u64 chunk = scan_chunk(n);
+  const u32 G = scan_blocks(n, chunk);
   if (!tb.branch) {
-    hipLaunchKernelGGL(k_lz_apply<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, cin,
-                       s->coef[s->cur], tb, s->cnt);
+    hipLaunchKernelGGL(k_lz_apply<W>, dim3(G), dim3(kLB), 0, s->stream, P, s->cap, n, chunk, cin,
+                       s->coef[s->cur], tb, s->cnt, s->bsum);
     LHIPCHK(hipGetLastError());
     return PPVM_OK;
   }
   double* cout = s->coef[s->cur ^ 1];
-  hipLaunchKernelGGL(k_lz_apply<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, cin, cout, tb, s->cnt);
+  hipLaunchKernelGGL(k_lz_apply<W>, dim3(G), dim3(kLB), 0, s->stream, P, s->cap, n, chunk, cin, cout, tb, s->cnt,
+                     s->bsum);
   LHIPCHK(hipGetLastError());
   u64 M = 0;
   int rc;
-  if ((rc = lscan<u32>(s, s->cnt, n, s->off, &M))) return rc;
+  if ((rc = lscan_offsets(s, G, &M))) return rc;
   if (M == 0) {  // the planes stay in buffer cur: only the coefficient buffers trade places
     std::swap(s->coef[s->cur], s->coef[s->cur ^ 1]);
     return PPVM_OK;
@@ -692,8 +778,9 @@ int run_op(ppvm_lossy_sum* s, const LTable& tb) {
     if ((rc = lgrow(s, n + M))) return rc;
     return run_op<W>(s, tb);
   }
-  const u64 tmask = s->tcap - 1;
-  if ((rc = lindex_w<W>(s, P, s->cap, n, s->table, s->tcap))) return rc;
+  hipLaunchKernelGGL(k_lz_scan<u32>, dim3(G), dim3(kLB), 0, s->stream, s->cnt, n, chunk, s->bsum, s->off);
+  u64 tmask = 0;
+  if ((rc = lindex_w<W>(s, P, s->cap, n, s->table, s->tcap, &tmask))) return rc;
   hipLaunchKernelGGL(k_lz_child_flags<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->cnt,
                      s->off, s->table, tmask, s->flag);
   LHIPCHK(hipGetLastError());
@@ -718,14 +805,22 @@ int run_truncate(ppvm_lossy_sum* s) {
   if (n == 0) return PPVM_OK;
   const bool coef_off = !(s->tau > 0.0);
   if (coef_off && s->maxw == ~0ull && s->maxlw == ~0ull) return PPVM_OK;
-  hipLaunchKernelGGL(k_lz_keep<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, s->planes[s->cur], s->cap, n,
-                     s->coef[s->cur], s->tau, s->maxw, s->maxlw, s->preserve, s->n_preserve, s->off);
+  const u64 chunk = scan_chunk(n);
+  const u32 G = scan_blocks(n, chunk);
+  LHIPCHK(hipMemsetAsync(s->dtot, 0, 8, s->stream));
+  hipLaunchKernelGGL(k_lz_keep<W>, dim3(G), dim3(kLB), 0, s->stream, s->planes[s->cur], s->cap, n, chunk,
+                     s->coef[s->cur], s->tau, s->maxw, s->maxlw, s->preserve, s->n_preserve, s->off, s->bsum,
+                     s->dtot);
   LHIPCHK(hipGetLastError());
-  u64 tot = 0;
-  int rc;
-  if ((rc = lscan<u64>(s, s->off, n, s->off2, &tot))) return rc;
+  LHIPCHK(hipMemcpyAsync(s->htot, s->dtot, 8, hipMemcpyDeviceToHost, s->stream));
+  LHIPCHK(hipStreamSynchronize(s->stream));
+  const u64 tot = *s->htot;
   const u64 kept = tot & 0xffffffffull, restored = tot >> 32;
-  if (kept == n) return PPVM_OK;
+  if (kept == n) return PPVM_OK;  // nothing dropped
+  u64 check = 0;
+  int rc;
+  if ((rc = lscan_offsets(s, G, &check))) return rc;
+  hipLaunchKernelGGL(k_lz_scan<u64>, dim3(G), dim3(kLB), 0, s->stream, s->off, n, chunk, s->bsum, s->off2);
   const int o = s->cur ^ 1;
   hipLaunchKernelGGL(k_lz_compact<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
                      s->planes[o], s->coef[o], s->cap, n, s->off, s->off2, kept);
@@ -760,10 +855,11 @@ int run_overlap(ppvm_lossy_sum* a, ppvm_lossy_sum* b, double* out) {
   *out = 0.0;
   if (a->len == 0 || b->len == 0) return PPVM_OK;
   int rc;
-  if ((rc = lindex_w<W>(a, a->planes[a->cur], a->cap, a->len, a->table, a->tcap))) return rc;
+  u64 tmask = 0;
+  if ((rc = lindex_w<W>(a, a->planes[a->cur], a->cap, a->len, a->table, a->tcap, &tmask))) return rc;
   const u32 G = std::min<u32>(kLRedGrid, lgrid(b->len));
   hipLaunchKernelGGL(k_lz_overlap<W>, dim3(G), dim3(kLB), 0, a->stream, a->planes[a->cur], a->coef[a->cur], a->cap,
-                     a->table, a->tcap - 1, b->planes[b->cur], b->coef[b->cur], b->cap, b->len, a->partials);
+                     a->table, tmask, b->planes[b->cur], b->coef[b->cur], b->cap, b->len, a->partials);
   LHIPCHK(hipGetLastError());
   std::vector<double> part(G);
   LHIPCHK(hipMemcpyAsync(part.data(), a->partials, G * 8, hipMemcpyDeviceToHost, a->stream));

@@ -18,6 +18,9 @@ Configurations (sizes as measured):
                  observable (seed 13), threshold 1e-4, max weight 12.
   batched_chains 256 chains, L~U{32..128}, h~U[0.5,1.5], steps~U{5..30}
                  (seed 21), J=1, dt=0.1, Z at L//2, threshold 1e-5.
+  lossy_lattice_tfim  the lattice TFIM on lossy words, 4 steps, loss 0.01 on
+                 every qubit after each step, reset-loss at readout, max loss
+                 weight 2.
 """
 
 from __future__ import annotations
@@ -145,6 +148,39 @@ def lattice_tfim(L: int = 10, J: float = 1.0, h: float = 3.04, dt: float = 0.05,
     return Workload("lattice_tfim", n, min_abs_coeff, max_pauli_weight, [(_z_string(n, [oq]), 1.0)],
                     prog.array(), marks,
                     dict(L=L, J=J, h=h, dt=dt, steps=steps, obs_qubit=oq))
+
+
+def lossy_lattice_tfim(L: int = 10, J: float = 1.0, h: float = 3.04, dt: float = 0.05, steps: int = 4,
+                       p_loss: float = 0.01, obs: tuple[int, int] = (5, 5), min_abs_coeff: float = 1e-5,
+                       max_pauli_weight: int | None = 10, max_loss_weight: int = 2) -> Workload:
+    """The lattice TFIM on lossy words: each forward step is followed by a loss
+    channel (probability p_loss) on every qubit, and the readout resets lost
+    qubits to |0> (reset-loss on every qubit at the end of the circuit, i.e.
+    first in the backward program).  Terms with more than max_loss_weight lost
+    qubits are truncated."""
+    n = L * L
+    horiz, vert = lattice_bonds(L)
+    fwd: list[tuple] = []
+    starts: set[int] = set()
+    for _ in range(steps):
+        starts.add(len(fwd))
+        fwd += [(O.RZZ, a, b, 2.0 * J * dt, None) for a, b in horiz + vert]
+        fwd += [(O.RX, q, 0, 2.0 * h * dt, None) for q in range(n)]
+        fwd += [(O.LOSS, q, 0, 0.0, [p_loss]) for q in range(n)]
+    fwd += [(O.RESET_LOSS, q, 0, 0.0, None) for q in range(n)]
+    prog = O.Program()
+    marks: list[int] = []
+    for fi in range(len(fwd) - 1, -1, -1):
+        code, q0, q1, th, pp = fwd[fi]
+        prog.add(code, q0, q1, theta=th, p=pp)
+        prog.truncate()
+        if fi in starts:
+            marks.append(len(prog) - 1)
+    oq = L * obs[0] + obs[1]
+    return Workload("lossy_lattice_tfim", n, min_abs_coeff, max_pauli_weight, [(_z_string(n, [oq]), 1.0)],
+                    prog.array(), marks,
+                    dict(L=L, J=J, h=h, dt=dt, steps=steps, p_loss=p_loss, obs_qubit=oq,
+                         max_loss_weight=max_loss_weight))
 
 
 def heavy_hex_graph(seed: int = 7) -> tuple[int, list[tuple[int, int]]]:
