@@ -69,6 +69,28 @@ def test_argument_validation_before_device():
         _native.check(L.ppvm_truncate(None))
 
 
+def test_lossy_entry_points_without_device():
+    L = _native.lib()
+    h = ctypes.c_void_p()
+    mx = (1 << 64) - 1
+    assert L.ppvm_lossy_create(0, 1e-4, mx, mx, 0, ctypes.byref(h)) == _native.PPVM_EINVAL
+    assert L.ppvm_lossy_create(600, 1e-4, mx, mx, 0, ctypes.byref(h)) == _native.PPVM_EINVAL
+    assert L.ppvm_lossy_create(10, -1.0, mx, mx, 0, ctypes.byref(h)) == _native.PPVM_EINVAL
+    assert L.ppvm_lossy_apply(None, None, 0) == _native.PPVM_EINVAL
+    if not _has_device():
+        assert L.ppvm_lossy_create(20, 1e-4, mx, 2, 0, ctypes.byref(h)) == _native.PPVM_ENODEV
+        assert b"device" in L.ppvm_lossy_last_error()
+
+
+def test_lossy_words_layout_roundtrip():
+    strings = ["XLZI" * 20, "I" * 79 + "L", "L" * 80, "_".join("IXYZ" * 20)]
+    x, z, l = O.pack_lossy_strings(strings, 80)
+    assert x.shape == (4, 2) and not np.any(l & (x | z))
+    assert O.unpack_lossy_words(x, z, l, 80) == [s.replace("_", "") for s in strings]
+    with pytest.raises(ValueError):
+        O.pack_lossy_strings(["LQ"], 2)
+
+
 def test_words_layout_roundtrip():
     strings = ["XYZI" * 20, "I" * 79 + "Y", "Z" * 80]
     x, z = O.pack_strings(strings, 80)
