@@ -613,9 +613,60 @@ def run_lossy(args) -> dict:
     return res
 
 
+# Wall times recorded in crates/ppvm-pauli-sum/examples/trotter_storage_cliff.rs:16-20
+# for this circuit through the reference's Python API (hardware not stated).
+REFERENCE_EXAMPLE_MS = {32: 223.0, 40: 544.0, 48: 1044.0, 56: 1842.0, 64: 401.0}
+
+
+def run_example(args) -> dict:
+    """The reference's own timed example (examples/trotter_storage_cliff.rs):
+    noisy 1D TFIM, N = 32..64, 20 steps, threshold 1e-6, truncation after every
+    call.  GPU time per propagation (best of --steps runs, handle created and
+    loaded before the clock), the oracle's single-thread time on this host,
+    bit-identity of the final terms, and the reference's recorded times."""
+    from oracle.pyoracle import OracleSum
+    from ppvm_amd import workloads as WL
+    rows = []
+    for n in (32, 40, 48, 56, 64):
+        w = WL.noisy_chain_tfim(n)
+        times = []
+        for i in range(args.warmup + args.steps):
+            g = _make(w, 1 << 18)
+            t0 = time.perf_counter()
+            g.apply(w.program)
+            g.synchronize()
+            if i >= args.warmup:
+                times.append(time.perf_counter() - t0)
+        gx, gz, gc = g._export()
+        u = g._h.counters()["term_updates"]
+        row = {"n": n, "gpu_ms": min(times) * 1e3, "terms": int(len(gc)), "term_updates": u,
+               "reference_ms_recorded": REFERENCE_EXAMPLE_MS[n]}
+        if not args.no_cpu:
+            o = OracleSum(w.n_qubits, w.min_abs_coeff, w.max_pauli_weight)
+            o.add_strings([t for t, _ in w.observable], [c for _, c in w.observable])
+            t0 = time.perf_counter()
+            o.apply(w.program)
+            row["oracle_ms"] = (time.perf_counter() - t0) * 1e3
+            ox, oz, oc = o.export()
+            row["identical"] = bool(np.array_equal(ox, gx) and np.array_equal(oz, gz)
+                                    and np.array_equal(oc.view(np.uint64), gc.view(np.uint64)))
+        _log(f"example n={n}: {row}")
+        rows.append(row)
+    return {
+        "metric": "ms per propagation, reference example trotter_storage_cliff (noisy 1D TFIM)",
+        "unit": "ms", "n_gpus": 1, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "noisy_chain_tfim: sum_i Z_i, J=1, h=1, dt=0.1, 20 steps, pauli_error 2.5e-5 "
+                               "x3 around every gate, threshold 1e-6, truncate after every call"},
+        "rows": rows,
+        "note": "reference_ms_recorded: wall times written in the reference example's comments "
+                "(Python API, hardware not stated)",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity", "lossy"], default="lattice")
+    ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity", "lossy", "example"],
+                    default="lattice")
     ap.add_argument("--verify", type=int, default=16, help="batched: instances re-run on the engine")
     ap.add_argument("--cpu-instances", type=int, default=256, help="batched: at most this many instances on the CPU")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="batched: CPU baseline time budget (s)")
@@ -630,7 +681,7 @@ def main() -> int:
                                    "this build (--gpus must be 1)", "n_gpus": args.gpus}))
         return 2
     res = {"lattice": run, "batched": run_batched, "surrogate": run_surrogate,
-           "parity": run_parity, "lossy": run_lossy}[args.mode](args)
+           "parity": run_parity, "lossy": run_lossy, "example": run_example}[args.mode](args)
     print(json.dumps(res), flush=True)
     return 0
 

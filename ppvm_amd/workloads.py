@@ -116,6 +116,31 @@ def chain_tfim(n: int = 20, J: float = 1.0, h: float = 1.0, dt: float = 0.1, ste
                     dict(n=n, J=J, h=h, dt=dt, steps=steps, obs=obs))
 
 
+def noisy_chain_tfim(n: int, J: float = 1.0, h: float = 1.0, steps: int = 20, dt: float = 0.1,
+                     min_abs_coeff: float = 1e-6, noise: float = 1e-4) -> Workload:
+    """The circuit of crates/ppvm-pauli-sum/examples/trotter_storage_cliff.rs:44-95
+    (the reference's only recorded timings): observable sum_i Z_i; per step,
+    for every qubit pauli_error(i, noise/4 each) then rx(i, h dt), then for
+    every bond pauli_error(i+1), pauli_error(i), rzz(i, i+1, J dt); truncation
+    (threshold only) after every call.  The calls are applied as written, in
+    the reference's order."""
+    p3 = [noise / 4.0] * 3
+    prog = O.Program()
+    marks: list[int] = []
+    for _ in range(steps):
+        for i in range(n):
+            prog.add(O.PAULI_ERROR, i, p=p3).truncate()
+            prog.add(O.RX, i, theta=dt * h).truncate()
+        for i in range(n - 1):
+            prog.add(O.PAULI_ERROR, i + 1, p=p3).truncate()
+            prog.add(O.PAULI_ERROR, i, p=p3).truncate()
+            prog.add(O.RZZ, i, i + 1, theta=dt * J).truncate()
+        marks.append(len(prog) - 1)
+    obs = [(_z_string(n, [i]), 1.0) for i in range(n)]
+    return Workload("noisy_chain_tfim", n, min_abs_coeff, None, obs, prog.array(), marks,
+                    dict(n=n, J=J, h=h, steps=steps, dt=dt, noise=noise))
+
+
 def lattice_bonds(L: int):
     horiz = [(L * r + c, L * r + c + 1) for r in range(L) for c in range(L - 1)]
     vert = [(L * r + c, L * (r + 1) + c) for r in range(L - 1) for c in range(L)]

@@ -65,7 +65,7 @@ def _load(g, o, words, coefs):
 
 @pytest.mark.parametrize("n,seed,tau,maxw,maxlw", [
     (6, 0, 0.0, None, None), (9, 1, 1e-4, None, 2), (12, 2, 1e-6, 7, None),
-    (70, 3, 1e-5, 8, 3), (130, 4, 1e-5, None, 2)])
+    (70, 3, 1e-5, 8, 3), (130, 4, 1e-5, None, 2), (300, 5, 1e-5, 10, 2)])
 def test_random_lossy_circuits(n, seed, tau, maxw, maxlw):
     rng = np.random.default_rng(seed)
     support = np.array(sorted(rng.choice(n, min(n, 8), replace=False)))
@@ -183,3 +183,29 @@ def test_errors():
         LossyPauliSum.new(2, "ZQ")
     with pytest.raises(TypeError):
         LossyPauliSum.new(2, "ZZ").overlap(PauliSum.new(2, "ZZ"))
+
+
+def test_empty_sum_and_export_capacity():
+    import ctypes
+
+    from ppvm_amd import _native
+    g = _handle(5, 0.5)
+    o = LossyOracleSum(5, 0.5)
+    _load(g, o, ["ZLIII", "XIIII"], [0.1, 0.2])
+    prog = O.Program().truncate().add(O.RESET_LOSS, 0).add(O.CORRELATED_LOSS, 1, 2, p=[0.1, 0.1, 0.1])
+    prog.add(O.RXX, 0, 1, theta=0.4).add(O.LOSS, 3, p=[0.2]).truncate()
+    g.apply(prog.array())
+    o.apply(prog.array())
+    assert len(g) == 0 and len(o) == 0
+    assert g.overlap_with_zero() == 0.0 and g.trace("Z?*") == 0.0
+    x, z, l, c = g.export()
+    assert len(c) == 0
+    # a buffer smaller than the sum: nothing written, the length reported, EINVAL
+    h = _handle(3)
+    _load(h, LossyOracleSum(3), ["ZLI", "IIZ"], [1.0, 2.0])
+    buf = np.zeros((1, 1), dtype=np.uint64)
+    cbuf = np.zeros(1)
+    got = ctypes.c_size_t()
+    rc = _native.lib().ppvm_lossy_export_terms(h.h, _native.u64p(buf), _native.u64p(buf), _native.u64p(buf),
+                                                _native.f64p(cbuf), 1, ctypes.byref(got))
+    assert rc == _native.PPVM_EINVAL and got.value == 2
