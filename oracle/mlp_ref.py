@@ -1,6 +1,6 @@
 """numpy reference of the surrogate MLP (forward, MSE loss, backward).
 
-TEST INFRASTRUCTURE ONLY: the checker for the GPU MLP kernels' gradients.
+TEST INFRASTRUCTURE ONLY: the reference computation the GPU MLP kernels' gradients are checked against.
 It follows the GPU's precision plan exactly where the plan is defined by
 rounding points: weights and activations rounded to BF16 (round to nearest
 even), products and sums in FP64 (the GPU accumulates exact BF16 products in

@@ -2,7 +2,7 @@
 //
 // TEST INFRASTRUCTURE ONLY (parity oracle + CPU baseline).  Only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this library,
-// and only as the checker or the timed CPU baseline.
+// and only to check results against or as the timed CPU baseline.
 //
 // Reference files restated here (paths relative to the reference repository):
 //   crates/ppvm-pauli-sum/src/sum/data.rs:192-311   scale / map_insert / map_add / truncate

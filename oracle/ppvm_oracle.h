@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle and the CPU
  * baseline of the measurement harness.  Only tests/, __graft_entry__.smoke()
- * and bench.py's cpu_baseline leg may load it, and only as the checker (or the
- * timed CPU baseline) — never as part of the product path.
+ * and bench.py's cpu_baseline leg may load it, and only to check results
+ * against (or as the timed CPU baseline) — never as part of the product path.
  *
  * It restates, in plain C++, the algorithm of the files listed in
  * ppvm_oracle.cpp: an insertion-ordered hash map (IndexMap semantics), rotation

@@ -2,7 +2,7 @@
 
 TEST INFRASTRUCTURE ONLY: the parity oracle and the CPU baseline.  Only
 ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline leg may
-import this module, and only as the checker or the timed CPU baseline.
+import this module, and only to check results against or as the timed CPU baseline.
 """
 
 from __future__ import annotations
