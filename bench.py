@@ -613,6 +613,48 @@ def run_lossy(args) -> dict:
     return res
 
 
+def run_configs(args) -> dict:
+    """The other single-sum configurations, each a full backward propagation:
+    the 20-qubit chain TFIM and the 127-qubit random Clifford + rz circuit.
+    GPU time (best of --steps runs on a handle created before the clock), term
+    updates, the single-thread oracle's time on this host, and bit-identity of
+    the final terms."""
+    from ppvm_amd import workloads as WL
+    rows = []
+    for w in (WL.chain_tfim(), WL.random_circuit()):
+        times = []
+        for i in range(args.warmup + args.steps):
+            g = _make(w, 1 << 18)
+            t0 = time.perf_counter()
+            g.apply(w.program)
+            g.synchronize()
+            if i >= args.warmup:
+                times.append(time.perf_counter() - t0)
+        gx, gz, gc = g._export()
+        u = g._h.counters()["term_updates"]
+        gpu_s = min(times)
+        row = {"workload": w.name, "n_qubits": w.n_qubits, "gates": w.n_gates, "gpu_ms": gpu_s * 1e3,
+               "term_updates": u, "value": u / gpu_s, "peak_live": g.stats()["peak_live"],
+               "final_terms": int(len(gc)), "overlap_with_zero": g.overlap_with_zero()}
+        if not args.no_cpu:
+            from oracle.pyoracle import OracleSum
+            o = OracleSum(w.n_qubits, w.min_abs_coeff, w.max_pauli_weight)
+            o.add_strings([t for t, _ in w.observable], [c for _, c in w.observable])
+            t0 = time.perf_counter()
+            o.apply(w.program)
+            cpu_s = time.perf_counter() - t0
+            ox, oz, oc = o.export()
+            row.update({"oracle_ms": cpu_s * 1e3, "cpu_value": o.term_updates() / cpu_s,
+                        "identical": bool(np.array_equal(ox, gx) and np.array_equal(oz, gz)
+                                          and np.array_equal(oc.view(np.uint64), gc.view(np.uint64)))})
+        _log(f"config {w.name}: {row}")
+        rows.append(row)
+    return {"metric": "term-updates/s per single-sum configuration", "unit": "term-updates/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic", "rows": rows,
+            "config": {"workload": "chain_tfim (n=20, J=1, h=1, dt=0.1, 10 steps, Z_10, threshold 1e-4); "
+                                   "random_circuit (127 qubits, 30 layers, threshold 1e-4, max weight 12)"}}
+
+
 # Wall times recorded in crates/ppvm-pauli-sum/examples/trotter_storage_cliff.rs:16-20
 # for this circuit through the reference's Python API (hardware not stated).
 REFERENCE_EXAMPLE_MS = {32: 223.0, 40: 544.0, 48: 1044.0, 56: 1842.0, 64: 401.0}
@@ -665,7 +707,7 @@ def run_example(args) -> dict:
 
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity", "lossy", "example"],
+    ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity", "lossy", "example", "configs"],
                     default="lattice")
     ap.add_argument("--verify", type=int, default=16, help="batched: instances re-run on the engine")
     ap.add_argument("--cpu-instances", type=int, default=256, help="batched: at most this many instances on the CPU")
@@ -681,7 +723,8 @@ def main() -> int:
                                    "this build (--gpus must be 1)", "n_gpus": args.gpus}))
         return 2
     res = {"lattice": run, "batched": run_batched, "surrogate": run_surrogate,
-           "parity": run_parity, "lossy": run_lossy, "example": run_example}[args.mode](args)
+           "parity": run_parity, "lossy": run_lossy, "example": run_example,
+           "configs": run_configs}[args.mode](args)
     print(json.dumps(res), flush=True)
     return 0
 
