@@ -16,6 +16,9 @@
 //   crates/ppvm-traits/src/map/hashmap.rs:40-54,108-127,170-179,242-259 IndexMap ops
 //   crates/ppvm-traits/src/traits/clifford.rs:117-165,270-341 Clifford bit maps
 //   crates/ppvm-pauli-word/src/phase/clifford.rs:16-180 Clifford sign rules
+//   crates/ppvm-pauli-word/src/loss/data.rs:33-316    lossy words (get / set / weights)
+//   crates/ppvm-pauli-sum/src/sum/noise.rs:159-274    loss, correlated loss, reset loss
+//   crates/ppvm-pauli-sum/src/strategy.rs:137-170     MaxLossWeight
 //
 // The map is insertion ordered (IndexMap): `+=` on a new key appends, `retain`
 // keeps order, Clifford re-insertion keeps order.  Floating point follows the
