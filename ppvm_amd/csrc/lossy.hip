@@ -668,13 +668,17 @@ int lgrow(ppvm_lossy_sum* s, u64 want) {
     return lfail(PPVM_ENOMEM, "device allocation of a lossy arena of %llu entries failed",
                  (unsigned long long)cap);
   }
-  if (s->len) {
-    for (u64 p = 0; p < np; ++p)
-      LHIPCHK(hipMemcpyAsync(t.planes[0] + p * cap, s->planes[s->cur] + p * s->cap, s->len * 8,
-                             hipMemcpyDeviceToDevice, s->stream));
-    LHIPCHK(hipMemcpyAsync(t.coef[0], s->coef[s->cur], s->len * 8, hipMemcpyDeviceToDevice, s->stream));
+  hipError_t e = hipSuccess;
+  for (u64 p = 0; p < np && s->len && e == hipSuccess; ++p)
+    e = hipMemcpyAsync(t.planes[0] + p * cap, s->planes[s->cur] + p * s->cap, s->len * 8, hipMemcpyDeviceToDevice,
+                       s->stream);
+  if (e == hipSuccess && s->len)
+    e = hipMemcpyAsync(t.coef[0], s->coef[s->cur], s->len * 8, hipMemcpyDeviceToDevice, s->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+  if (e != hipSuccess) {
+    lfree(&t);
+    return lfail(PPVM_EHIP, "copying the lossy arena failed: %s", hipGetErrorString(e));
   }
-  LHIPCHK(hipStreamSynchronize(s->stream));
   lfree(s);
   for (int b = 0; b < 2; ++b) {
     s->planes[b] = t.planes[b];
@@ -692,7 +696,7 @@ int lgrow(ppvm_lossy_sum* s, u64 want) {
   return PPVM_OK;
 }
 
-// Items per block of the chunked passes: 4096, doubled until the blocks fit bsum.
+// Items per block of the chunked passes: kLScanChunk, doubled until the blocks fit bsum.
 u64 scan_chunk(u64 n) {
   u64 chunk = kLScanChunk;
   while ((n + chunk - 1) / chunk > kLScanMaxBlocks) chunk *= 2;
