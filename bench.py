@@ -124,6 +124,15 @@ def measured_traffic(kernel: str):
     return rows[kernel][1] if kernel in rows else None
 
 
+# DevState counters that accumulate over a handle's life (the rest are levels)
+CUMULATIVE = ("term_updates", "gates", "rotations", "scanned", "wl_total", "pairs_total", "children_total",
+              "kills_total", "map_entries", "map_launches", "scan_planes", "transposed", "pair_fallback")
+
+
+def counter_delta(after: dict, before: dict) -> dict:
+    return {k: (after[k] - before[k]) if k in CUMULATIVE else after[k] for k in after}
+
+
 def run(args) -> dict:
     from ppvm_amd import _native
     from ppvm_amd import workloads as WL
@@ -132,7 +141,9 @@ def run(args) -> dict:
     _log(f"device: {info}")
     w = WL.lattice_tfim()
     W = 2
-    # warm-up: learn the peak size and term-update count, load the kernels
+    # warm-up: the first propagation grows the handle's arenas to the peak
+    # size and loads the kernels; every later propagation reuses the same
+    # arenas (reset = clear + upload of the one-term observable)
     t0 = time.perf_counter()
     g = _make(w)
     g.apply(w.program)
@@ -145,38 +156,39 @@ def run(args) -> dict:
     ovl = g.overlap_with_zero()
     n_final = len(g)
     _log(f"warm-up run: {warm_s:.2f}s U={U} peak_live={peak} final={n_final} cap={cap} overlap={ovl:.12f}")
-    del g
     for _ in range(max(0, args.warmup - 1)):
-        g = _make(w, cap)
+        g.reset()
         g.apply(w.program)
         g.synchronize()
-        del g
 
-    # timed region: K full propagations, each on a pre-sized handle
-    handles = [_make(w, cap) for _ in range(args.steps)]
-    for h in handles:
-        h.synchronize()
-    t0 = time.perf_counter()
-    for h in handles:
-        h.apply(w.program)
-        h.synchronize()
-    elapsed = time.perf_counter() - t0
-    for h in handles:
-        c = h._h.counters()
-        assert c["term_updates"] == U, "non-deterministic term-update count"
-        assert abs(h.overlap_with_zero() - ovl) <= 1e-10
-    del handles
+    # timed region: K full propagations on the pre-sized handle; each step's
+    # clock runs from a synchronised, freshly reset handle (observable already
+    # in HBM) to the synchronised end of the propagation
+    elapsed = 0.0
+    step_s = []
+    for _ in range(args.steps):
+        g.reset()
+        before = g._h.counters()["term_updates"]  # synchronises
+        t0 = time.perf_counter()
+        g.apply(w.program)
+        g.synchronize()
+        dt = time.perf_counter() - t0
+        elapsed += dt
+        step_s.append(dt)
+        assert g._h.counters()["term_updates"] - before == U, "non-deterministic term-update count"
+        assert abs(g.overlap_with_zero() - ovl) <= 1e-10
     value = U * args.steps / elapsed
     _log(f"timed: {args.steps} x {elapsed / args.steps:.3f}s  -> {value:.4e} term-updates/s")
 
     # profiled pass: per-kernel HIP-event timing on the handle's stream
-    g = _make(w, cap)
+    g.reset()
+    cb = g._h.counters()
     g._h.profile(True)
     g.apply(w.program)
     g.synchronize()
     ms, launches = g._h.profile_read()
-    cnt = g._h.counters()
-    del g
+    g._h.profile(False)
+    cnt = counter_delta(g._h.counters(), cb)
     pb = phase_bytes(cnt, W, w.n_qubits)
     kern = {}
     for i, nm in enumerate(KERNELS):
@@ -219,6 +231,7 @@ def run(args) -> dict:
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": elapsed / args.steps * 1e3,
+        "step_ms": {"min": min(step_s) * 1e3, "max": max(step_s) * 1e3},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -233,6 +246,7 @@ def run(args) -> dict:
             "final_terms": int(n_final),
             "overlap_with_zero": ovl,
             "device": info,
+            "arena_capacity": cap,
         },
         "roofline": roofline,
         "kernels": kern,
@@ -249,17 +263,22 @@ def run(args) -> dict:
         ref.apply(p.program)
         cpu_s = time.perf_counter() - t0
         cpu_rate = ref.term_updates() / cpu_s
-        gp = _make(p, cap)
-        gp.synchronize()
-        t0 = time.perf_counter()
-        gp.apply(p.program)
-        gp.synchronize()
-        gpu_s = time.perf_counter() - t0
+        # the GPU on the same prefix, same pre-sized handle (best of 3 runs)
+        gpu_s = None
+        for _ in range(3):
+            g.reset()
+            cb = g._h.counters()["term_updates"]
+            t0 = time.perf_counter()
+            g.apply(p.program)
+            g.synchronize()
+            dt = time.perf_counter() - t0
+            gpu_s = dt if gpu_s is None else min(gpu_s, dt)
+            gu = g._h.counters()["term_updates"] - cb
+        gp = g
         gx, gz, gc = gp._export()
         ox, oz, oc = ref.export()
         same = (len(gc) == len(oc) and np.array_equal(gx, ox) and np.array_equal(gz, oz)
                 and np.array_equal(gc.view(np.uint64), oc.view(np.uint64)))
-        gu = gp._h.counters()["term_updates"]
         # (ii) sharded multi-thread variant on the host's cores (same prefix)
         threads = max(1, min(16, os.cpu_count() or 1))
         shd = OracleSum(p.n_qubits, p.min_abs_coeff, p.max_pauli_weight)
@@ -285,6 +304,7 @@ def run(args) -> dict:
             "host_cpus": os.cpu_count(),
             "cpu_model": cpu_model(),
             "gpu_same_prefix": gu / gpu_s,
+            "speedup_same_prefix": (gu / gpu_s) / best["value"],
         }
         cpu_rate = best["value"]
         result["parity"] = {
@@ -294,7 +314,11 @@ def run(args) -> dict:
             "term_updates_equal": bool(gu == ref.term_updates()),
             "overlap_abs_diff": abs(gp.overlap_with_zero() - ref.overlap_with_zero()),
         }
-        result["speedup_vs_cpu"] = value / cpu_rate
+        # like with like: GPU and CPU rates over the same prefix of the workload
+        result["speedup_vs_cpu"] = (gu / gpu_s) / cpu_rate
+        result["speedup_vs_cpu_basis"] = (f"GPU and better CPU baseline on the same {ps}-step prefix; "
+                                          "the full-run GPU rate (value) over the prefix CPU rate is "
+                                          f"{value / cpu_rate:.1f}x")
         _log(f"cpu: single {single['value']:.4e}/s, sharded x{threads} {shd_rate:.4e}/s over {ps} steps; "
              f"parity={same}")
     return result

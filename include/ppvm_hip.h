@@ -136,6 +136,13 @@ int ppvm_set_preserve(ppvm_handle h, const uint64_t* x, const uint64_t* z, size_
 int ppvm_add_terms(ppvm_handle h, const uint64_t* x, const uint64_t* z, const double* c,
                    size_t count);
 
+/* Empties the sum and keeps its arenas (capacity, preserve set, truncation
+ * strategy and counters are unchanged); replaces `ps.data_mut().clear()`
+ * (ACMap::clear, crates/ppvm-traits/src/traits/map.rs:23).  Queued gates are
+ * discarded.  Lets a caller rerun a circuit on a pre-sized handle without
+ * reallocating. */
+int ppvm_clear(ppvm_handle h);
+
 /* Clifford gates on a target list; replace Clifford / CliffordExtensions and
  * their *_many forms (crates/ppvm-traits/src/traits/clifford.rs:30-79,175-218). */
 int ppvm_x(ppvm_handle h, const uint32_t* targets, size_t n);

@@ -57,6 +57,7 @@ def _declare(L):
     d("ppvm_words", _H, _u32p)
     d("ppvm_set_preserve", _H, _u64p, _u64p, ctypes.c_size_t)
     d("ppvm_add_terms", _H, _u64p, _u64p, _f64p, ctypes.c_size_t)
+    d("ppvm_clear", _H)
     for g in ("x", "y", "z", "h", "s", "s_dag", "sqrt_x", "sqrt_x_dag", "sqrt_y", "sqrt_y_dag",
               "cnot", "cz", "cy"):
         d("ppvm_" + g, _H, _u32p, ctypes.c_size_t)
@@ -177,6 +178,9 @@ class Handle:
         if not (len(x) == len(z) == len(c)):
             raise ValueError("Initial terms and coefficients need to be of same length!")
         check(lib().ppvm_add_terms(self.h, u64p(x), u64p(z), f64p(c), len(c)))
+
+    def clear(self):
+        check(lib().ppvm_clear(self.h))
 
     def set_preserve(self, x: np.ndarray, z: np.ndarray):
         x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, self.W)

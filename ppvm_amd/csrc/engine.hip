@@ -1416,6 +1416,28 @@ int ppvm_add_terms(ppvm_handle h, const uint64_t* x, const uint64_t* z, const do
   return PPVM_OK;
 }
 
+int ppvm_clear(ppvm_handle h) {
+  if (!h) return fail(PPVM_EINVAL, "null handle");
+  if (h->broken) return fail(PPVM_ESTATE, "handle is unusable after an earlier device error");
+  // queued and issued-but-unchecked ops act on terms that are being dropped:
+  // discard them (an overflow they raised needs no replay either)
+  h->pending.clear();
+  h->skip_run = 0;
+  h->log.clear();
+  h->issued_since_sync = 0;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, h->stream, &h->st->overflow, 1ull, 0ull);
+  HIPCHK(hipGetLastError());
+  const std::vector<u64> keys;
+  const std::vector<double> cs;
+  int rc = upload(h, keys, cs);  // zero entries; synchronises the stream
+  if (rc) {
+    h->broken = true;
+    return rc;
+  }
+  h->clean = true;
+  return PPVM_OK;
+}
+
 int ppvm_x(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_X, t, n, 0.0); }
 int ppvm_y(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_Y, t, n, 0.0); }
 int ppvm_z(ppvm_handle h, const uint32_t* t, size_t n) { return gate_1q(h, PPVM_OP_Z, t, n, 0.0); }

@@ -150,6 +150,12 @@ class PauliSum:
                    max_pauli_weight=max_pauli_weight, max_loss_weight=max_loss_weight,
                    preserve_strings=preserve_strings, capacity_hint=capacity_hint)
 
+    def reset(self) -> None:
+        """Restore the initial terms on the same device arenas (no reallocation):
+        the sum is cleared and the constructor's terms are added again."""
+        self._h.clear()
+        self._h.add_words(*self._pack(self.initial_terms), np.asarray(self.coefficients, dtype=np.float64))
+
     # ---- bookkeeping --------------------------------------------------------------
     def __len__(self) -> int:
         return len(self._h)
