@@ -214,6 +214,14 @@ int ppvm_apply(ppvm_handle h, const ppvm_op* ops, size_t n);
 
 /* Wait for all queued work; check device-side overflow and regrow if needed. */
 int ppvm_synchronize(ppvm_handle h);
+/* Order-independent digest of the live terms, for comparing a sum with one
+ * held by another backend the way PartialEq compares two sums (IndexMap
+ * equality ignores order; crates/ppvm-pauli-sum/src/sum/data.rs:349-356)
+ * without copying the terms out: out = {count, sum_k d(k), sum_k e(k, c_k)}
+ * modulo 2^64, where d chains the SplitMix64 finaliser m over the key words
+ * (d = m(...m(m(0x9E3779B97F4A7C15 ^ x_0) ^ x_1)... ^ z_{W-1}), x words then
+ * z words, W = ceil(n_qubits/64)) and e(k, c) = m(d(k) ^ bits of c). */
+int ppvm_fingerprint(ppvm_handle h, uint64_t out[3]);
 int ppvm_get_stats(ppvm_handle h, ppvm_stats* out);
 /* Device counters for the measurement harness, in this order: term_updates,
  * gates, rotations, scanned entries, anticommuting (worklist) terms, merged

@@ -1190,6 +1190,32 @@ uint64_t ppvo_export(ppvo_handle h, uint64_t* x, uint64_t* z, double* c, size_t 
 }
 uint64_t ppvo_term_updates(ppvo_handle h) { return h->term_updates; }
 
+// Order-independent digest of the term set (IndexMap equality ignores order):
+// per term, k = mix(...mix(mix(seed ^ x_0) ^ x_1)... ^ z_{W-1}) over the x then z
+// words, and kc = mix(k ^ coefficient bits); out = {count, sum k, sum kc}, sums
+// modulo 2^64.  mix is the SplitMix64 finaliser.
+static inline uint64_t fp_mix(uint64_t v) {
+  v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+  v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+  return v ^ (v >> 31);
+}
+void ppvo_fingerprint(ppvo_handle h, uint64_t out[3]) {
+  uint64_t ks = 0, kcs = 0;
+  const size_t n = h->len();
+  for (size_t i = 0; i < n; ++i) {
+    const uint64_t* k = h->key(i);
+    uint64_t v = 0x9E3779B97F4A7C15ull;
+    for (uint32_t p = 0; p < h->NW; ++p) v = fp_mix(v ^ k[p]);  // x, z (, loss) words
+    uint64_t cb;
+    std::memcpy(&cb, &h->coef[i], 8);
+    ks += v;
+    kcs += fp_mix(v ^ cb);
+  }
+  out[0] = n;
+  out[1] = ks;
+  out[2] = kcs;
+}
+
 ppvo_handle ppvo_lossy_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_pauli_weight,
                               uint64_t max_loss_weight) {
   ppvo_sum* s = ppvo_create(n_qubits, min_abs_coeff, max_pauli_weight);

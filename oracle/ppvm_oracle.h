@@ -47,6 +47,9 @@ uint64_t ppvo_len(ppvo_handle h);
 /* Writes min(cap, len) terms in insertion order; returns len. */
 uint64_t ppvo_export(ppvo_handle h, uint64_t* x, uint64_t* z, double* c, size_t cap);
 uint64_t ppvo_term_updates(ppvo_handle h);
+/* Order-independent digest of the live terms: {count, sum of key digests,
+ * sum of key-and-coefficient-bit digests} (modulo 2^64). */
+void ppvo_fingerprint(ppvo_handle h, uint64_t out[3]);
 /* Replays `ops` and, after every op whose index is listed in `marks`
  * (ascending), records len and overlap_with_zero into out_len/out_ovl.
  * Returns the number of marks recorded. */

@@ -62,6 +62,8 @@ def lib():
         L.ppvo_export.argtypes = [ctypes.c_void_p, _u64p, _u64p, _f64p, ctypes.c_size_t]
         L.ppvo_term_updates.restype = ctypes.c_uint64
         L.ppvo_term_updates.argtypes = [ctypes.c_void_p]
+        L.ppvo_fingerprint.restype = None
+        L.ppvo_fingerprint.argtypes = [ctypes.c_void_p, _u64p]
         L.ppvo_run_marked.restype = ctypes.c_size_t
         L.ppvo_run_marked.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, _u64p,
                                       ctypes.c_size_t, _u64p, _f64p]
@@ -169,6 +171,13 @@ class OracleSum:
 
     def term_updates(self) -> int:
         return int(lib().ppvo_term_updates(self._h))
+
+    def fingerprint(self) -> tuple:
+        """(count, key digest sum, key+coefficient digest sum): an
+        order-independent digest of the live terms."""
+        out = np.zeros(3, dtype=np.uint64)
+        lib().ppvo_fingerprint(self._h, _p64(out))
+        return tuple(int(v) for v in out)
 
     def export(self):
         n = len(self)

@@ -78,6 +78,7 @@ def _declare(L):
       ctypes.POINTER(ctypes.c_size_t))
     d("ppvm_apply", _H, ctypes.c_void_p, ctypes.c_size_t)
     d("ppvm_synchronize", _H)
+    d("ppvm_fingerprint", _H, _u64p)
     d("ppvm_get_stats", _H, ctypes.POINTER(PpvmStats))
     d("ppvm_get_counters", _H, _u64p)
     d("ppvm_profile_enable", _H, ctypes.c_int)
@@ -259,6 +260,13 @@ class Handle:
         got = ctypes.c_size_t()
         check(lib().ppvm_export_terms(self.h, u64p(x), u64p(z), f64p(c), n, ctypes.byref(got)))
         return x, z, c
+
+    def fingerprint(self) -> tuple:
+        """(count, key digest sum, key+coefficient digest sum) of the live
+        terms, computed on the device; order-independent."""
+        out = np.zeros(3, dtype=np.uint64)
+        check(lib().ppvm_fingerprint(self.h, u64p(out)))
+        return tuple(int(v) for v in out)
 
     def stats(self) -> dict:
         s = PpvmStats()

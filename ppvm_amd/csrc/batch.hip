@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -413,9 +414,15 @@ __device__ double b_overlap(const BInst& I, double* ldsd) {
   return block_sum_d(acc, ldsd);
 }
 
+// Table slots carry a 24-bit generation tag ((u64)gen << 40); one generation
+// per rotation.  When an instance's generation reaches gen_limit (at most
+// kBGenLimit) its table is cleared and the count restarts, so a stale slot can
+// never carry the current tag.
+constexpr u32 kBGenLimit = (1u << 24) - 2;
+
 template <int W>
 __global__ __launch_bounds__(kBMaxThreads) void k_batch(BInst* insts, u64* sups, const u32* order, u32 n_order,
-                                                        u32* queue, u32 op_lo, u32 op_hi) {
+                                                        u32* queue, u32 op_lo, u32 op_hi, u32 gen_limit) {
   __shared__ u32 lds[kBMaxThreads / 64 + 1];
   __shared__ double ldsd[kBMaxThreads];
   __shared__ u64 lsup[(kBMaxThreads / 64) * 16];
@@ -453,6 +460,11 @@ __global__ __launch_bounds__(kBMaxThreads) void k_batch(BInst* insts, u64* sups,
         // a rotation that anticommutes with no live term changes nothing, and
         // the truncation fused with it has nothing to drop on a clean state
         if (b_may_act(s_sup, W, op) || (trunc && !I.clean)) {
+          if (I.gen >= gen_limit) {
+            for (u64 t = threadIdx.x; t < I.tcap; t += blockDim.x) I.table[t] = 0ull;
+            __syncthreads();
+            I.gen = 0u;
+          }
           I.gen = I.gen + 1u;
           ok = b_rotate<W>(I, op, trunc, lds, s_sup, lsup);
           if (!ok) break;
@@ -604,10 +616,20 @@ int pull_instances(ppvm_batch* b) {
   return PPVM_OK;
 }
 
+// Generation limit of the instance tables: kBGenLimit, or a smaller value from
+// PPVM_BATCH_GEN_LIMIT (at least 1; lets a test reach the table clear).
+u32 batch_gen_limit() {
+  const char* e = std::getenv("PPVM_BATCH_GEN_LIMIT");
+  if (!e || !*e) return kBGenLimit;
+  char* end = nullptr;
+  const unsigned long v = std::strtoul(e, &end, 10);
+  return (end && *end == '\0' && v >= 1 && v < kBGenLimit) ? (u32)v : kBGenLimit;
+}
+
 template <int W>
 int launch_batch(ppvm_batch* b, u32 threads, u32 blocks, u32 n_order, u32 seg, u32 op_lo, u32 op_hi) {
   hipLaunchKernelGGL(k_batch<W>, dim3(blocks), dim3(threads), 0, b->stream, b->d_inst, b->d_sup, b->d_order, n_order,
-                     b->d_queue + seg, op_lo, op_hi);
+                     b->d_queue + seg, op_lo, op_hi, batch_gen_limit());
   BCHK(hipGetLastError());
   return PPVM_OK;
 }
@@ -677,6 +699,97 @@ void noise_table(const ppvm_op& op, double f[16], u32& mask, u32& b_out) {
   }
 }
 
+u32 engine_words(u32 w) { return w <= 1 ? 1 : w <= 2 ? 2 : w <= 4 ? 4 : 8; }
+
+int validate_instance(const ppvm_batch_instance& s, size_t i) {
+  if (s.n_qubits == 0 || s.n_qubits > 512) return bfail(PPVM_EINVAL, "instance %zu: n_qubits %u out of range", i, s.n_qubits);
+  if (s.n_terms && (!s.x || !s.z || !s.c)) return bfail(PPVM_EINVAL, "instance %zu: null term arrays", i);
+  if (s.n_ops && !s.ops) return bfail(PPVM_EINVAL, "instance %zu: null program", i);
+  if (!(s.min_abs_coeff >= 0.0)) return bfail(PPVM_EINVAL, "instance %zu: bad min_abs_coeff", i);
+  for (u32 j = 0; j < s.n_ops; ++j) {
+    const ppvm_op& op = s.ops[j];
+    const bool two = op.code == PPVM_OP_CNOT || op.code == PPVM_OP_CZ || op.code == PPVM_OP_CY ||
+                     op.code == PPVM_OP_RXX || op.code == PPVM_OP_RYY || op.code == PPVM_OP_RZZ ||
+                     op.code == PPVM_OP_DEPOLARIZE2 || op.code == PPVM_OP_PAULI_ERROR2;
+    if (op.code == PPVM_OP_NOP || op.code == PPVM_OP_TRUNCATE) continue;
+    if (op.code == PPVM_OP_AMPLITUDE_DAMPING)
+      return bfail(PPVM_EINVAL, "instance %zu op %u: amplitude damping is not supported in batched mode", i, j);
+    if (!is_rot(op.code) && !is_map(op.code)) return bfail(PPVM_EINVAL, "instance %zu op %u: unknown code %u", i, j, op.code);
+    if (op.q0 >= s.n_qubits || (two && (op.q1 >= s.n_qubits || op.q1 == op.q0)))
+      return bfail(PPVM_EINVAL, "instance %zu op %u: qubit out of range", i, j);
+  }
+  return PPVM_OK;
+}
+
+// Appends the instance's device ops (and noise factor tables, 16 doubles
+// each) to ops / facs; returns the number of marked ops.
+u32 translate_program(const ppvm_batch_instance& s, std::vector<BOp>& ops, std::vector<double>& facs) {
+  u32 ntab = 0, marks = 0;
+  bool prev_gate = false;
+  for (u32 j = 0; j < s.n_ops; ++j) {
+    const ppvm_op& op = s.ops[j];
+    BOp d{};
+    d.code = op.code;
+    d.a = op.q0;
+    d.b = op.q1;
+    if (is_rot(op.code)) {
+      sincos(op.theta, &d.sn, &d.cs);  // as the single-sum engine (one libm sincos)
+    } else if (op.code >= PPVM_OP_PAULI_ERROR && op.code <= PPVM_OP_PAULI_ERROR2) {
+      double f[16];
+      u32 mask, bq;
+      noise_table(op, f, mask, bq);
+      d.table = ntab++;
+      d.mask = mask;
+      d.b = bq;
+      facs.insert(facs.end(), f, f + 16);
+    }
+    const bool next_trunc = j + 1 < s.n_ops && s.ops[j + 1].code == PPVM_OP_TRUNCATE;
+    const bool is_gate = is_rot(op.code) || is_map(op.code);
+    const bool marked = (op.reserved & 1u) != 0u;
+    // a truncate right after a gate runs inside that gate, unless the state
+    // between the two is recorded
+    const bool fuse = is_gate && next_trunc && !marked;
+    if (fuse) d.flags |= 1u;
+    if (op.code == PPVM_OP_TRUNCATE && prev_gate) d.flags |= 4u;  // fused into the gate before
+    if (marked) {
+      d.flags |= 2u;
+      ++marks;
+    }
+    prev_gate = fuse || (op.code == PPVM_OP_TRUNCATE && prev_gate && !marked);
+    ops.push_back(d);
+  }
+  return marks;
+}
+
+// Appends the instance's initial terms to kx / kz / kc in the engine layout
+// (W words per plane), merging repeated words with `+=`; returns the count.
+u64 dedupe_initial(const ppvm_batch_instance& s, u32 W, std::vector<u64>& kx, std::vector<u64>& kz,
+                   std::vector<double>& kc) {
+  const u32 wa = (s.n_qubits + 63) / 64;
+  const size_t first = kc.size();
+  for (u32 t = 0; t < s.n_terms; ++t) {
+    std::vector<u64> x(W, 0), z(W, 0);
+    for (u32 p = 0; p < wa; ++p) {
+      x[p] = s.x[(size_t)t * wa + p];
+      z[p] = s.z[(size_t)t * wa + p];
+    }
+    bool found = false;
+    for (size_t e = first; e < kc.size(); ++e) {
+      if (std::equal(x.begin(), x.end(), kx.begin() + e * W) && std::equal(z.begin(), z.end(), kz.begin() + e * W)) {
+        kc[e] += s.c[t];
+        found = true;
+        break;
+      }
+    }
+    if (!found) {
+      kx.insert(kx.end(), x.begin(), x.end());
+      kz.insert(kz.end(), z.begin(), z.end());
+      kc.push_back(s.c[t]);
+    }
+  }
+  return kc.size() - first;
+}
+
 }  // namespace
 
 extern "C" {
@@ -694,28 +807,12 @@ int ppvm_batch_create(const ppvm_batch_instance* inst, size_t n, ppvm_batch_hand
   }
   u32 wmax = 1;
   for (size_t i = 0; i < n; ++i) {
-    const ppvm_batch_instance& s = inst[i];
-    if (s.n_qubits == 0 || s.n_qubits > 512) return bfail(PPVM_EINVAL, "instance %zu: n_qubits %u out of range", i, s.n_qubits);
-    if (s.n_terms && (!s.x || !s.z || !s.c)) return bfail(PPVM_EINVAL, "instance %zu: null term arrays", i);
-    if (s.n_ops && !s.ops) return bfail(PPVM_EINVAL, "instance %zu: null program", i);
-    if (!(s.min_abs_coeff >= 0.0)) return bfail(PPVM_EINVAL, "instance %zu: bad min_abs_coeff", i);
-    for (u32 j = 0; j < s.n_ops; ++j) {
-      const ppvm_op& op = s.ops[j];
-      const bool two = op.code == PPVM_OP_CNOT || op.code == PPVM_OP_CZ || op.code == PPVM_OP_CY ||
-                       op.code == PPVM_OP_RXX || op.code == PPVM_OP_RYY || op.code == PPVM_OP_RZZ ||
-                       op.code == PPVM_OP_DEPOLARIZE2 || op.code == PPVM_OP_PAULI_ERROR2;
-      if (op.code == PPVM_OP_NOP || op.code == PPVM_OP_TRUNCATE) continue;
-      if (op.code == PPVM_OP_AMPLITUDE_DAMPING)
-        return bfail(PPVM_EINVAL, "instance %zu op %u: amplitude damping is not supported in batched mode", i, j);
-      if (!is_rot(op.code) && !is_map(op.code)) return bfail(PPVM_EINVAL, "instance %zu op %u: unknown code %u", i, j, op.code);
-      if (op.q0 >= s.n_qubits || (two && (op.q1 >= s.n_qubits || op.q1 == op.q0)))
-        return bfail(PPVM_EINVAL, "instance %zu op %u: qubit out of range", i, j);
-    }
-    const u32 w = (s.n_qubits + 63) / 64;
-    wmax = std::max(wmax, w);
+    int rc = validate_instance(inst[i], i);
+    if (rc) return rc;
+    wmax = std::max(wmax, (inst[i].n_qubits + 63) / 64);
   }
   ppvm_batch* b = new ppvm_batch();
-  b->W = wmax <= 1 ? 1 : wmax <= 2 ? 2 : wmax <= 4 ? 4 : 8;
+  b->W = engine_words(wmax);
   b->n = (u32)n;
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
     delete b;
@@ -733,83 +830,19 @@ int ppvm_batch_create(const ppvm_batch_instance* inst, size_t n, ppvm_batch_hand
     const ppvm_batch_instance& s = inst[i];
     op_off[i] = ops.size();
     fac_off[i] = facs.size() / 16;
-    u32 ntab = 0;
-    u32 marks = 0;
-    bool prev_gate = false;
-    for (u32 j = 0; j < s.n_ops; ++j) {
-      const ppvm_op& op = s.ops[j];
-      BOp d{};
-      d.code = op.code;
-      d.a = op.q0;
-      d.b = op.q1;
-      if (is_rot(op.code)) {
-        sincos(op.theta, &d.sn, &d.cs);  // as the single-sum engine (one libm sincos)
-      } else if (op.code >= PPVM_OP_PAULI_ERROR && op.code <= PPVM_OP_PAULI_ERROR2) {
-        double f[16];
-        u32 mask, bq;
-        noise_table(op, f, mask, bq);
-        d.table = ntab++;
-        d.mask = mask;
-        d.b = bq;
-        facs.insert(facs.end(), f, f + 16);
-      }
-      const bool next_trunc = j + 1 < s.n_ops && s.ops[j + 1].code == PPVM_OP_TRUNCATE;
-      const bool is_gate = is_rot(op.code) || is_map(op.code);
-      const bool marked = (op.reserved & 1u) != 0u;
-      // a truncate right after a gate runs inside that gate, unless the state
-      // between the two is recorded
-      const bool fuse = is_gate && next_trunc && !marked;
-      if (fuse) d.flags |= 1u;
-      if (op.code == PPVM_OP_TRUNCATE && prev_gate) d.flags |= 4u;  // fused into the gate before
-      if (marked) {
-        d.flags |= 2u;
-        ++marks;
-      }
-      prev_gate = fuse || (op.code == PPVM_OP_TRUNCATE && prev_gate && !marked);
-      ops.push_back(d);
-    }
     BInst& I = b->h_inst[i];
+    I.n_marks = translate_program(s, ops, facs);
     I.n_ops = s.n_ops;
     I.n_qubits = s.n_qubits;
     I.tau = s.min_abs_coeff;
     I.maxw = s.max_pauli_weight;
-    I.n_marks = marks;
-    (void)ntab;
   }
   // initial terms: deduplicate with `+=` semantics, engine word layout
   b->init_off.resize(n);
   b->n_qubits.resize(n);
   for (size_t i = 0; i < n; ++i) {
-    const ppvm_batch_instance& s = inst[i];
-    const u32 wa = (s.n_qubits + 63) / 64;
     b->init_off[i] = b->init_c.size();
-    std::vector<u64> kx, kz;
-    std::vector<double> kc;
-    for (u32 t = 0; t < s.n_terms; ++t) {
-      std::vector<u64> x(b->W, 0), z(b->W, 0);
-      for (u32 p = 0; p < wa; ++p) {
-        x[p] = s.x[(size_t)t * wa + p];
-        z[p] = s.z[(size_t)t * wa + p];
-      }
-      bool found = false;
-      for (size_t e = 0; e < kc.size(); ++e) {
-        if (std::equal(x.begin(), x.end(), kx.begin() + e * b->W) &&
-            std::equal(z.begin(), z.end(), kz.begin() + e * b->W)) {
-          kc[e] += s.c[t];
-          found = true;
-          break;
-        }
-      }
-      if (!found) {
-        kx.insert(kx.end(), x.begin(), x.end());
-        kz.insert(kz.end(), z.begin(), z.end());
-        kc.push_back(s.c[t]);
-      }
-    }
-    b->init_x.insert(b->init_x.end(), kx.begin(), kx.end());
-    b->init_z.insert(b->init_z.end(), kz.begin(), kz.end());
-    b->init_c.insert(b->init_c.end(), kc.begin(), kc.end());
-    b->h_inst[i].n_init = kc.size();
+    b->h_inst[i].n_init = dedupe_initial(inst[i], b->W, b->init_x, b->init_z, b->init_c);
   }
   int rc = PPVM_OK;
   if (hipMalloc(&b->d_inst, n * sizeof(BInst)) != hipSuccess ||

@@ -707,6 +707,14 @@ int refresh_support(ppvm_sum* s, u64 n) {
   return PPVM_OK;
 }
 
+template <int W>
+int launch_fingerprint(ppvm_sum* s, u32 G) {
+  hipLaunchKernelGGL(k_fingerprint<W>, dim3(G), dim3(kBlock), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
+                     s->cap, s->st, (u32)(s->op_seq & 1), s->Wa, s->part(5));
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
 int sync_check(ppvm_sum* s) {
   if (s->broken) return fail(PPVM_ESTATE, "handle is unusable after an earlier device error");
   int rc;
@@ -1581,6 +1589,33 @@ int ppvm_export_terms(ppvm_handle h, uint64_t* x, uint64_t* z, double* c, size_t
     }
     c[i] = cs[i];
   }
+  return PPVM_OK;
+}
+
+int ppvm_fingerprint(ppvm_handle h, uint64_t out[3]) {
+  if (!h || !out) return fail(PPVM_EINVAL, "null argument");
+  int rc;
+  if ((rc = ppvm_synchronize(h))) return rc;
+  const CountSlot c = h->st_host->slot[h->op_seq & 1];
+  const u32 G = (u32)grid_for(c.n_entries, (u64)kBlock * 8, 1024);  // 2 G <= kMaxParts
+  switch (h->W) {
+    case 1: rc = launch_fingerprint<1>(h, G); break;
+    case 2: rc = launch_fingerprint<2>(h, G); break;
+    case 4: rc = launch_fingerprint<4>(h, G); break;
+    default: rc = launch_fingerprint<8>(h, G); break;
+  }
+  if (rc) return rc;
+  std::vector<u64> part((size_t)2 * G);
+  HIPCHK(hipMemcpyAsync(part.data(), h->part(5), part.size() * 8, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  u64 ks = 0, kcs = 0;
+  for (u32 b = 0; b < G; ++b) {
+    ks += part[2 * (size_t)b];
+    kcs += part[2 * (size_t)b + 1];
+  }
+  out[0] = c.n_entries - c.n_dead;
+  out[1] = ks;
+  out[2] = kcs;
   return PPVM_OK;
 }
 

@@ -1021,6 +1021,42 @@ __global__ __launch_bounds__(kBlock) void k_support(const u64* __restrict__ plan
   }
 }
 
+// ---- k_fingerprint: order-independent digest of the live terms ------------------
+// Per live entry, k = fmix(...fmix(fmix(seed ^ x_0) ^ x_1)... ^ z_{wa-1}) over the
+// wa ABI words of each plane, kc = fmix(k ^ coefficient bits); per block the
+// wrap-around sums of k and kc go to out[2 block], out[2 block + 1] (the host
+// adds the blocks).  Integer sums, so the result does not depend on the order.
+__device__ __forceinline__ u64 fp_fmix(u64 v) {
+  v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+  v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+  return v ^ (v >> 31);
+}
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_fingerprint(const u64* __restrict__ planes,
+                                                        const double* __restrict__ coef, u64 cap,
+                                                        const DevState* __restrict__ st, u32 in_slot, u32 wa,
+                                                        u64* __restrict__ out) {
+  __shared__ u64 lds[kWaves];
+  const u64 n = st->slot[in_slot].n_entries;
+  u64 ks = 0, kcs = 0;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double c = coef[i];
+    if (is_dead(c)) continue;
+    u64 v = 0x9E3779B97F4A7C15ull;
+    for (u32 p = 0; p < wa; ++p) v = fp_fmix(v ^ planes[(u64)p * cap + i]);
+    for (u32 p = 0; p < wa; ++p) v = fp_fmix(v ^ planes[(u64)(W + p) * cap + i]);
+    ks += v;
+    kcs += fp_fmix(v ^ (u64)__double_as_longlong(c));
+  }
+  const u64 a = block_sum_all(ks, lds);
+  const u64 b = block_sum_all(kcs, lds);
+  if (threadIdx.x == 0) {
+    out[2 * (u64)blockIdx.x] = a;
+    out[2 * (u64)blockIdx.x + 1] = b;
+  }
+}
+
 // ---- k_skip_gates: rotations that act on no term --------------------------------
 // The host drops a run of rotations whose generators anticommute with no Pauli
 // present anywhere in the sum (judged from the union of the terms' x and z

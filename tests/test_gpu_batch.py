@@ -103,3 +103,19 @@ def test_hybrid_routes_large_instances_to_engine():
         lens, ovl = o.run_marked(w.program, w.layer_marks)
         assert np.array_equal(np.asarray(r.lens[i], dtype=np.uint64), lens)
         assert np.max(np.abs(np.asarray(r.overlaps[i]) - ovl)) <= 1e-10
+
+
+@pytest.mark.parametrize("limit", ["1", "3"])
+def test_batch_table_generation_wrap(monkeypatch, limit):
+    """With the generation limit forced down (PPVM_BATCH_GEN_LIMIT), every
+    instance's hash table is cleared and its generation restarted after every
+    `limit` rotations; results stay bit-identical to the oracle's, across a
+    second run that continues from the generations the first left behind."""
+    from ppvm_amd.batch import Batch
+    monkeypatch.setenv("PPVM_BATCH_GEN_LIMIT", limit)
+    ws = _small_chains(6, seed=13)
+    b = Batch(ws)
+    b.run(threads=256)
+    _check_batch(b, ws)
+    b.run(threads=256)
+    _check_batch(b, ws)
