@@ -11,7 +11,6 @@ reverse circuit order (Heisenberg picture).
 
 from __future__ import annotations
 
-import re
 from collections.abc import Iterable, Sequence
 from typing import Any
 
@@ -20,29 +19,47 @@ import numpy as np
 from . import _native
 from .ops import pack_lossy_strings, pack_strings, unpack_lossy_words, unpack_words, weight_of
 
-_COMPACT_RE = re.compile(r"^([IXYZ]\d+)+$")
-_COMPACT_TOKEN_RE = re.compile(r"([IXYZ])(\d+)")
-
 _TWO_Q_ORDER = ("IX", "IY", "IZ", "XI", "XX", "XY", "XZ", "YI", "YX", "YY", "YZ", "ZI", "ZX",
                 "ZY", "ZZ")
 
 
+def _compact_tokens(s: str):
+    """Split the compact form ``"X3Z5"`` (a Pauli letter and a qubit index,
+    repeated, nothing else) into ``[("X", 3), ("Z", 5)]``; None if ``s`` is not
+    in that form (then it is a full string)."""
+    tokens, i, n = [], 0, len(s)
+    if n == 0:
+        return None
+    while i < n:
+        letter = s[i]
+        if letter not in "IXYZ":
+            return None
+        j = i + 1
+        while j < n and s[j].isdecimal():
+            j += 1
+        if j == i + 1:
+            return None
+        tokens.append((letter, int(s[i + 1:j])))
+        i = j
+    return tokens
+
+
 def _parse_term(term, n_qubits: int):
-    """Full string, compact ``"X3Z5"`` form, or ``(string, coeff)`` tuple
-    (ppvm-python/src/ppvm/paulisum.py:20-38)."""
-    if isinstance(term, tuple):
-        s, coeff = term
-    else:
-        s, coeff = term, 1.0
-    if _COMPACT_RE.match(s):
-        chars = ["I"] * n_qubits
-        for pauli, idx_str in _COMPACT_TOKEN_RE.findall(s):
-            idx = int(idx_str)
-            if idx >= n_qubits:
-                raise ValueError(f"Qubit index {idx} out of range for {n_qubits}-qubit system.")
-            chars[idx] = pauli
-        s = "".join(chars)
-    return s, float(coeff)
+    """A term given as a full string, in the compact form, or as a
+    ``(string, coefficient)`` tuple, as the reference's PauliSum.new accepts
+    (ppvm-python/src/ppvm/paulisum.py:20-38): returns (full string, coefficient).
+    In the compact form, unnamed qubits are I and a later letter for the same
+    qubit wins."""
+    text, coeff = term if isinstance(term, tuple) else (term, 1.0)
+    tokens = _compact_tokens(text)
+    if tokens is not None:
+        full = ["I"] * n_qubits
+        for letter, q in tokens:
+            if q >= n_qubits:
+                raise ValueError(f"Qubit index {q} out of range for {n_qubits}-qubit system.")
+            full[q] = letter
+        text = "".join(full)
+    return text, float(coeff)
 
 
 def _is_sequence(obj: Any) -> bool:

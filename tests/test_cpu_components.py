@@ -104,3 +104,22 @@ def test_shipped_sweep_grid_matches_oracle_prefix():
             if li % 3 == 2:
                 vals.append(o.overlap_with_zero())
         assert np.max(np.abs(np.array(vals) - d["grid"][i, :10])) <= 1e-10
+
+
+def test_term_parser_forms():
+    """PauliSum.new's term forms (ppvm-python/src/ppvm/paulisum.py:20-38):
+    full strings pass through, the compact form names qubits by index (a later
+    letter for the same qubit wins, unnamed qubits are I), tuples carry a
+    coefficient, and an index past the register raises."""
+    import pytest
+    from ppvm_amd.paulisum import _parse_term
+    assert _parse_term("XYZI", 4) == ("XYZI", 1.0)
+    assert _parse_term("X3Z0", 5) == ("ZIIXI", 1.0)
+    assert _parse_term(("Z10", 0.5), 12) == ("I" * 10 + "ZI", 0.5)
+    assert _parse_term("X1Y1", 3) == ("IYI", 1.0)
+    assert _parse_term("I0", 2) == ("II", 1.0)
+    # not the compact form: kept as given (the constructor checks the length)
+    assert _parse_term("X9Y", 3) == ("X9Y", 1.0)
+    assert _parse_term("X", 3) == ("X", 1.0)
+    with pytest.raises(ValueError, match="Qubit index 3 out of range for 3-qubit system."):
+        _parse_term("Z3", 3)
