@@ -814,8 +814,9 @@ int apply_sharded(Sum& src, const ppvm_op* ops, size_t nops, int threads) {
 //   sum/rot1.rs:9-128, sum/rot2.rs:17-184 (a lost qubit: no rotation, or the
 //     one-qubit rotation on the surviving qubit),
 //   sum/noise.rs:17-274 (channels; loss, correlated loss, reset loss),
-// with the scratch-vector merge of map_insert / map_insert_multiple
-// (sum/data.rs:181-226, map/hashmap.rs:170-179,242-259).
+// with the scratch-vector merge of map_insert (sum/data.rs:204-226,
+// map/hashmap.rs:170-179) and the aux-map merge of map_insert_multiple
+// (sum/data.rs:181-189,234-247, map/hashmap.rs:181-194,302-308).
 enum LPauli { LP_I = 0, LP_X = 1, LP_Z = 2, LP_Y = 3, LP_L = 4 };
 
 // LossyPauliWord::get / set (loss/data.rs:232-316)
@@ -831,8 +832,45 @@ void lset(const Sum& s, uint64_t* k, uint32_t q, int p) {
   w = p == LP_L ? (w | m) : (w & ~m);
 }
 
-// map_insert_multiple: f(key, &coef, kids, kid_coefs) returns the number of
-// children (0..3); children are merged with `+=` after the pass, in order.
+// map_insert_multiple (sum/data.rs:234-247, map/hashmap.rs:181-194, consume
+// sum/data.rs:181-189 and map/hashmap.rs:302-308): f(key, &coef, kids,
+// kid_coefs) returns the number of children (0..3).  The children are merged
+// among themselves into an empty insertion-ordered aux map (`+=` in parent,
+// then slot order); then the smaller map is consumed into the larger one with
+// `+=`: if aux holds more keys than the sum, the sum's entries are added into
+// aux in order and aux becomes the sum (aux order first, then the sum's other
+// keys), otherwise aux's entries are added into the sum.  A key in both ends
+// with p + S (S the children's sum), in either case.
+template <class F>
+void lmap_insert_multiple(Sum& s, F f) {
+  Sum aux;
+  aux.n = s.n;
+  aux.W = s.W;
+  aux.NW = s.NW;
+  aux.lossy = s.lossy;
+  aux.rebuild_index();
+  std::vector<uint64_t> kids(3 * s.NW);
+  double kv[3];
+  const size_t n = s.len();
+  s.term_updates += n;
+  for (size_t i = 0; i < n; ++i) {
+    const int m = f(s.key(i), s.coef[i], kids.data(), kv);
+    for (int j = 0; j < m; ++j) aux.add_assign(&kids[j * s.NW], kv[j]);
+  }
+  if (aux.len() > s.len()) {
+    for (size_t i = 0; i < s.len(); ++i) aux.add_assign(s.key(i), s.coef[i]);
+    s.keys.swap(aux.keys);
+    s.coef.swap(aux.coef);
+    s.hashes.swap(aux.hashes);
+    s.rebuild_index();
+  } else {
+    for (size_t j = 0; j < aux.len(); ++j) s.add_assign(aux.key(j), aux.coef[j]);
+  }
+}
+
+// map_insert (sum/data.rs:204-226): f(key, &coef, kids, kid_coefs) returns the
+// number of children (0 or 1 here); children are collected in a scratch
+// vector and merged into the sum with `+=` after the pass, in order.
 template <class F>
 void lmap_insert(Sum& s, F f) {
   std::vector<uint64_t> ck;
@@ -1063,9 +1101,9 @@ int lossy_apply_one(Sum& s, const ppvm_op& op) {
       });
       return 0;
     }
-    case PPVM_OP_CORRELATED_LOSS: {  // noise.rs:182-248
+    case PPVM_OP_CORRELATED_LOSS: {  // noise.rs:182-248 (map_insert_multiple)
       const double* p = op.p;
-      lmap_insert(s, [&](const uint64_t* k, double& v, uint64_t* kid, double* kv) {
+      lmap_insert_multiple(s, [&](const uint64_t* k, double& v, uint64_t* kid, double* kv) {
         const int pa = lget(s, k, a), pb = lget(s, k, b);
         if (pa == LP_L && pb == LP_L) {
           const double v_il = v * p[2];

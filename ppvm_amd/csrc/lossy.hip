@@ -99,6 +99,7 @@ struct LTable {
   u32 two;        // 1: local state = 5 * state(a) + state(b)
   u32 branch;     // 1: the op emits children (map_insert); 0: scale / map_add
   u32 moves;      // 1: some parent changes its local state (Cliffords)
+  u32 multi;      // 1: map_insert_multiple merge (children summed first, then added once)
   LEntry e[25];
   LPre pre[25];
 };
@@ -358,9 +359,106 @@ __global__ __launch_bounds__(kLB) void k_lz_merge(const u64* __restrict__ planes
     Contrib c[3];
     const int m = gather<W>(planes, cap, table, tmask, tb, k, t, c);
     if (m == 0) continue;
-    double v = cout[i];
-    for (int r = 0; r < m; ++r) v = __dadd_rn(v, __dmul_rn(cin[c[r].p], tb.e[c[r].s].cf[c[r].j]));
-    cout[i] = v;
+    if (tb.multi) {
+      // map_insert_multiple: the children meet in the aux map first, and their
+      // sum is added to the entry once (sum/data.rs:234-247, consume :181-189)
+      double sum = __dmul_rn(cin[c[0].p], tb.e[c[0].s].cf[c[0].j]);
+      for (int r = 1; r < m; ++r) sum = __dadd_rn(sum, __dmul_rn(cin[c[r].p], tb.e[c[r].s].cf[c[r].j]));
+      cout[i] = __dadd_rn(cout[i], sum);
+    } else {
+      // map_insert: the scratch vector's children are added one by one
+      // (sum/data.rs:204-226)
+      double v = cout[i];
+      for (int r = 0; r < m; ++r) v = __dadd_rn(v, __dmul_rn(cin[c[r].p], tb.e[c[r].s].cf[c[r].j]));
+      cout[i] = v;
+    }
+  }
+}
+
+// map_insert_multiple, order (sum/data.rs:181-189): when the aux map of
+// distinct children holds more keys than the sum, the result lists aux's keys
+// in first-occurrence order, then the sum's keys that received no child, in
+// their order.  The three kernels below build that order into the other arena.
+//
+// Entries that received no child: flag[e] = 1.
+template <int W>
+__global__ __launch_bounds__(kLB) void k_lz_unhit(const u64* __restrict__ planes, u64 cap, u64 n, LTable tb,
+                                                  const u32* __restrict__ table, u64 tmask, u32* __restrict__ flag) {
+  for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) {
+    u64 k[3 * W];
+    lload<W>(planes, cap, i, k);
+    const u32 t = local_of<W>(k, tb);
+    u32 f = 1u;
+    if (tb.pre[t].n) {
+      Contrib c[3];
+      f = gather<W>(planes, cap, table, tmask, tb, k, t, c) == 0 ? 1u : 0u;
+    }
+    flag[i] = f;
+  }
+}
+
+// Children that are the first contributor of their word: flag[child] = 1.
+template <int W>
+__global__ __launch_bounds__(kLB) void k_lz_first(const u64* __restrict__ planes, u64 cap, u64 n, LTable tb,
+                                                  const u32* __restrict__ cnt, const u64* __restrict__ off,
+                                                  const u32* __restrict__ table, u64 tmask, u32* __restrict__ flag) {
+  for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) {
+    const u32 nch = cnt[i];
+    if (nch == 0u) continue;
+    u64 k[3 * W];
+    lload<W>(planes, cap, i, k);
+    const LEntry& e = tb.e[local_of<W>(k, tb)];
+    const u64 base = off[i];
+    for (u32 j = 0; j < nch; ++j) {
+      u64 T[3 * W];
+#pragma unroll
+      for (int q = 0; q < 3 * W; ++q) T[q] = k[q];
+      put_local<W>(T, tb, e.cst[j]);
+      Contrib c[3];
+      const int m = gather<W>(planes, cap, table, tmask, tb, T, e.cst[j], c);
+      flag[base + j] = (m > 0 && c[0].p == i && c[0].j == j) ? 1u : 0u;
+    }
+  }
+}
+
+// Writes the reordered sum into (qp, qc): the child word of first-occurrence
+// rank r at r (its merged value: the entry's, or the new entry's at n + its
+// rank among the new entries), then the entries without children at A + their
+// rank among them (urank: exclusive ranks, R of them in all).
+template <int W>
+__global__ __launch_bounds__(kLB) void k_lz_reorder(const u64* __restrict__ planes, u64 cap, u64 n, LTable tb,
+                                                    const u32* __restrict__ cnt, const u64* __restrict__ off,
+                                                    const u32* __restrict__ table, u64 tmask,
+                                                    const u32* __restrict__ first, const u64* __restrict__ arank,
+                                                    const u32* __restrict__ isnew, const u64* __restrict__ nrank,
+                                                    const u64* __restrict__ urank, u64 R, u64 A,
+                                                    const double* __restrict__ cval, u64* __restrict__ qp,
+                                                    double* __restrict__ qc) {
+  for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) {
+    u64 k[3 * W];
+    lload<W>(planes, cap, i, k);
+    // exclusive ranks of the entries without a child: entry i has none iff the rank steps after it
+    const bool unhit = (i + 1 < n ? urank[i + 1] : R) != urank[i];
+    if (unhit) {
+      const u64 dst = A + urank[i];
+      lstore<W>(qp, cap, dst, k);
+      qc[dst] = cval[i];
+    }
+    const u32 nch = cnt[i];
+    if (nch == 0u) continue;
+    const LEntry& e = tb.e[local_of<W>(k, tb)];
+    const u64 base = off[i];
+    for (u32 j = 0; j < nch; ++j) {
+      if (!first[base + j]) continue;
+      u64 T[3 * W];
+#pragma unroll
+      for (int q = 0; q < 3 * W; ++q) T[q] = k[q];
+      put_local<W>(T, tb, e.cst[j]);
+      const u64 src = isnew[base + j] ? n + nrank[base + j] : lfind<W>(planes, cap, table, tmask, T);
+      const u64 dst = arank[base + j];
+      lstore<W>(qp, cap, dst, T);
+      qc[dst] = cval[src];
+    }
   }
 }
 
@@ -618,6 +716,8 @@ struct ppvm_lossy_sum {
   u64* off2 = nullptr;    // per entry: truncation offsets
   u32* flag = nullptr;    // per child: creates a new entry
   u64* pos = nullptr;     // per child: rank among new entries
+  u32* flag2 = nullptr;   // per child: first contributor of its word (map_insert_multiple order)
+  u64* pos2 = nullptr;    // per child: rank among those
   u32* table = nullptr;   // word index
   u64* bsum = nullptr;    // scan block sums
   u64* dtot = nullptr;    // scan total (device)
@@ -636,12 +736,12 @@ void lfree(ppvm_lossy_sum* s) {
     s->planes[b] = nullptr;
     s->coef[b] = nullptr;
   }
-  void* bufs[] = {s->cnt, s->off, s->off2, s->flag, s->pos, s->table};
+  void* bufs[] = {s->cnt, s->off, s->off2, s->flag, s->pos, s->flag2, s->pos2, s->table};
   for (void* p : bufs)
     if (p) (void)hipFree(p);
   s->cnt = nullptr;
-  s->off = s->off2 = s->pos = nullptr;
-  s->flag = s->table = nullptr;
+  s->off = s->off2 = s->pos = s->pos2 = nullptr;
+  s->flag = s->flag2 = s->table = nullptr;
 }
 
 // Allocates arenas of `cap` entries, keeping the current `len` entries.
@@ -661,7 +761,8 @@ int lgrow(ppvm_lossy_sum* s, u64 want) {
   const u64 tcap = lpow2(2 * cap);
   ok = ok && hipMalloc(&t.cnt, cap * 4) == hipSuccess && hipMalloc(&t.off, cap * 8) == hipSuccess &&
        hipMalloc(&t.off2, cap * 8) == hipSuccess && hipMalloc(&t.flag, cap * 4) == hipSuccess &&
-       hipMalloc(&t.pos, cap * 8) == hipSuccess && hipMalloc(&t.table, tcap * 4) == hipSuccess;
+       hipMalloc(&t.pos, cap * 8) == hipSuccess && hipMalloc(&t.flag2, cap * 4) == hipSuccess &&
+       hipMalloc(&t.pos2, cap * 8) == hipSuccess && hipMalloc(&t.table, tcap * 4) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
     lfree(&t);
@@ -689,6 +790,8 @@ int lgrow(ppvm_lossy_sum* s, u64 want) {
   s->off2 = t.off2;
   s->flag = t.flag;
   s->pos = t.pos;
+  s->flag2 = t.flag2;
+  s->pos2 = t.pos2;
   s->table = t.table;
   s->cap = cap;
   s->tcap = tcap;
@@ -796,6 +899,39 @@ int run_op(ppvm_lossy_sum* s, const LTable& tb) {
     hipLaunchKernelGGL(k_lz_child_write<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->cnt,
                        s->off, s->table, tmask, s->flag, s->pos, cin, cout);
   LHIPCHK(hipGetLastError());
+  if (tb.multi && NEW) {
+    // map_insert_multiple: aux holds NEW + (entries that received a child)
+    // keys; if that exceeds n, aux's order comes first (sum/data.rs:181-189).
+    // Buffers: flag2 = entries without a child, pos2 = their ranks; then
+    // flag2 = first contributors, off2 = their ranks (aux order).
+    hipLaunchKernelGGL(k_lz_unhit<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->table, tmask,
+                       s->flag2);
+    LHIPCHK(hipGetLastError());
+    u64 R = 0;
+    if ((rc = lscan<u32>(s, s->flag2, n, s->pos2, &R))) return rc;
+    const u64 A = NEW + (n - R);
+    if (A > n) {
+      hipLaunchKernelGGL(k_lz_first<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->cnt, s->off,
+                         s->table, tmask, s->flag2);
+      LHIPCHK(hipGetLastError());
+      u64 A2 = 0;
+      if ((rc = lscan<u32>(s, s->flag2, M, s->off2, &A2))) return rc;
+      if (A2 != A)
+        return lfail(PPVM_EHIP, "map_insert_multiple order: %llu first children, %llu expected",
+                     (unsigned long long)A2, (unsigned long long)A);
+      // cout holds every merged value; the old coefficients (cin's buffer) are
+      // no longer read, so they take the reordered values
+      u64* qp = s->planes[s->cur ^ 1];
+      double* qc = s->coef[s->cur];
+      hipLaunchKernelGGL(k_lz_reorder<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->cnt, s->off,
+                         s->table, tmask, s->flag2, s->off2, s->flag, s->pos, s->pos2, R, A, cout, qp, qc);
+      LHIPCHK(hipGetLastError());
+      s->cur ^= 1;                           // planes[cur] = qp
+      std::swap(s->coef[0], s->coef[1]);     // coef[cur] = qc
+      s->len = n + NEW;
+      return PPVM_OK;
+    }
+  }
   std::swap(s->coef[s->cur], s->coef[s->cur ^ 1]);
   s->len = n + NEW;
   return PPVM_OK;
@@ -1106,6 +1242,9 @@ int build_table(const ppvm_op& op, LTable& tb) {
     }
     if (e.nch) tb.branch = 1;
   }
+  // the correlated loss channel goes through map_insert_multiple (noise.rs:193),
+  // every other branching op through map_insert
+  tb.multi = c == PPVM_OP_CORRELATED_LOSS ? 1u : 0u;
   if (tb.branch) {
     for (u32 t = 0; t < ns; ++t) {
       const LEntry& e = tb.e[t];

@@ -209,3 +209,40 @@ def test_empty_sum_and_export_capacity():
     rc = _native.lib().ppvm_lossy_export_terms(h.h, _native.u64p(buf), _native.u64p(buf), _native.u64p(buf),
                                                 _native.f64p(cbuf), 1, ctypes.byref(got))
     assert rc == _native.PPVM_EINVAL and got.value == 2
+
+
+@pytest.mark.parametrize("case", ["aux_smaller", "aux_larger", "single_ll", "random_lost"])
+def test_correlated_loss_merge_order_and_association(case):
+    """correlated_loss_channel goes through map_insert_multiple
+    (crates/ppvm-pauli-sum/src/sum/data.rs:234-247): children are summed among
+    themselves, added to an existing word once (p + S), and when the distinct
+    children outnumber the sum's words the children's order comes first.  GPU
+    words, order and coefficient bits equal the oracle's in every case."""
+    rng = np.random.default_rng(7)
+    pr = [0.3, 0.1, 0.7]
+    if case == "aux_smaller":
+        words, coefs = ["II", "IL", "LI", "LL"], [1e16, 4.0, 4.0, 1.3]
+        n = 2
+    elif case == "aux_larger":
+        words, coefs = ["LL", "IL"], [2.0, 0.5]
+        n = 2
+    elif case == "single_ll":
+        words, coefs = ["LL"], [1.0]
+        n = 2
+    else:
+        # many words with both target qubits lost: three distinct children each
+        n = 12
+        words = _words(rng, n, 300, list(range(2, n)))
+        words = sorted({w[:3] + "L" + w[4:6] + "L" + w[7:] for w in words})
+        coefs = list(rng.uniform(-1, 1, len(words)))
+    g, o = _handle(n), LossyOracleSum(n, 0.0, None, None)
+    _load(g, o, words, coefs)
+    a, b = (0, 1) if n == 2 else (3, 6)
+    prog = O.Program().add(O.CORRELATED_LOSS, a, b, 0.0, pr).array()
+    g.apply(prog)
+    o.apply(prog)
+    _same(g, o, case)
+    # and once more on the result (the first order decides the second's input)
+    g.apply(prog)
+    o.apply(prog)
+    _same(g, o, case + " twice")

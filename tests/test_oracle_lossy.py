@@ -109,14 +109,54 @@ def test_correlated_loss_known_answers():
     assert s.terms() == [("ZL", 1.0 - p[2]), ("ZI", p[1])]
     s = run(lossy(2, ["LZ"]), (O.CORRELATED_LOSS, 0, 1, 0.0, p))
     assert s.terms() == [("LZ", 1.0 - p[2]), ("IZ", p[1])]
+    # three distinct children against one entry: aux is the larger map, so the
+    # children come first and LL (unchanged by the channel) is appended after
+    # them (sum/data.rs:181-189)
     s = run(lossy(2, ["LL"]), (O.CORRELATED_LOSS, 0, 1, 0.0, p))
-    assert s.terms() == [("LL", 1.0), ("IL", p[2]), ("LI", p[2]), ("II", p[0])]
+    assert s.terms() == [("IL", p[2]), ("LI", p[2]), ("II", p[0]), ("LL", 1.0)]
     # tests/loss.rs:354-400 and test_loss.py::test_correlated_loss_ghz
     for obs, expect in (("ZZ", 1.0), ("XX", 0.9)):
         s = lossy(2, [obs], tau=1e-10)
         run(s, (O.RESET_LOSS, 0), (O.RESET_LOSS, 1), (O.CORRELATED_LOSS, 0, 1, 0.0, [0.1, 0.0, 0.0]),
             (O.CNOT, 0, 1), (O.H, 0))
         assert abs(s.overlap_with_zero() - expect) < 1e-9
+
+
+def test_correlated_loss_merges_children_before_the_entry():
+    """map_insert_multiple (sum/data.rs:234-247): a word's children are summed
+    in parent then slot order in the aux map, and that sum is added to the
+    existing coefficient once: p + ((c1 + c2) + c3), not ((p + c1) + c2) + c3.
+    Here II receives p1 b (from IL), p1 e (from LI) and p0 d (from LL); the
+    values are chosen so that the two associations differ in the last bit.
+    Four entries against three distinct children: the entries keep their order."""
+    pr = [0.3, 0.1, 0.7]
+    a, b, d, e = 1e16, 4.0, 1.3, 4.0
+    s = run(lossy(2, {"II": a, "IL": b, "LI": e, "LL": d}), (O.CORRELATED_LOSS, 0, 1, 0.0, pr))
+    ap = a * (1.0 - pr[1] * 2.0 - pr[0])  # II itself is in the qubit subspace on both
+    c1, c2, c3 = b * pr[1], e * pr[1], d * pr[0]
+    want_ii = ap + ((c1 + c2) + c3)
+    assert want_ii != ((ap + c1) + c2) + c3  # the case tells the two orders apart
+    t = s.terms()
+    assert [k for k, _ in t] == ["II", "IL", "LI", "LL"]
+    got = dict(t)
+    assert got["II"] == want_ii
+    assert got["IL"] == b * (1.0 - pr[2]) + d * pr[2]
+    assert got["LI"] == e * (1.0 - pr[2]) + d * pr[2]
+    assert got["LL"] == d
+
+
+def test_correlated_loss_larger_aux_puts_children_first():
+    # two entries, three distinct children (one of them an existing key):
+    # the result is aux's order (IL, LI, II), then the entries not among the
+    # children (LL), and IL = children's sum + its own value
+    pr = [0.2, 0.1, 0.4]
+    s = run(lossy(2, {"LL": 2.0, "IL": 0.5}), (O.CORRELATED_LOSS, 0, 1, 0.0, pr))
+    t = s.terms()
+    assert [k for k, _ in t] == ["IL", "LI", "II", "LL"]
+    got = dict(t)
+    assert got["IL"] == 2.0 * pr[2] + 0.5 * (1.0 - pr[2])
+    assert got["II"] == 2.0 * pr[0] + 0.5 * pr[1]
+    assert got["LL"] == 2.0
 
 
 def test_reset_loss_accumulates_in_order():
