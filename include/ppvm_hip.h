@@ -293,6 +293,24 @@ int ppvm_batch_marks(ppvm_batch_handle b, size_t i, uint64_t* lens, double* over
 int ppvm_batch_export(ppvm_batch_handle b, size_t i, uint64_t* x, uint64_t* z, double* c, size_t cap,
                       size_t* n);
 
+/* One instance through the batched kernel without allocating device memory.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * (code that must not call hipMalloc, such as serving loops or harnesses that
+ * forbid it).  Same results as ppvm_batch_create + ppvm_batch_run +
+ * ppvm_batch_results + ppvm_batch_export on a batch holding only `inst`
+ * (terms, order and coefficient bits; `capacity` sizes the arena as there).
+ * Every device buffer lives in a fixed 64 MiB scratch area per GPU that the
+ * library holds until the process ends; a run needing more (arena, hash
+ * table, program) fails with PPVM_EINVAL, and an arena overflow, which this
+ * entry point cannot grow, fails with PPVM_ENOMEM.  Inputs and outputs are
+ * host memory; work runs on the device's null stream and the call returns
+ * when it is done.  Calls on one GPU share the scratch area: calls that
+ * overlap (other host threads) race and must be serialised by the caller.
+ * threads: workgroup size (0: 256).  x, z, c receive the final terms
+ * (cap terms at most; *n = count); overlap_last and term_updates may be NULL. */
+int ppvm_batch_run1_noalloc(const ppvm_batch_instance* inst, uint32_t threads, uint64_t* x, uint64_t* z,
+                            double* c, size_t cap, size_t* n, double* overlap_last, uint64_t* term_updates);
+
 /* ---- lossy sums -----------------------------------------------------------------
  * Replaces PauliSum<LossyPauliWord> (crates/ppvm-pauli-word/src/loss/data.rs:33-45,
  * crates/ppvm-python-native/src/interface.rs:13-66, ppvm-python/src/ppvm/paulisum.py:420-520):

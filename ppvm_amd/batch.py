@@ -69,6 +69,9 @@ def _lib():
                                        ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
         L.ppvm_batch_export.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _u64p, _u64p, _f64p,
                                         ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.ppvm_batch_run1_noalloc.argtypes = [ctypes.POINTER(BatchInstance), ctypes.c_uint32, _u64p, _u64p,
+                                              _f64p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _f64p,
+                                              _u64p]
         _declared = True
     return L
 
@@ -180,6 +183,37 @@ class Batch:
         _check(_lib().ppvm_batch_export(self.h, i, x.ctypes.data_as(_u64p), z.ctypes.data_as(_u64p),
                                         c.ctypes.data_as(_f64p), n, ctypes.byref(got)))
         return x, z, c
+
+
+def run1_noalloc(w, capacity: int = 0, threads: int = 256, out_cap: int = 1 << 16):
+    """One workload through the batched kernel with no device allocation
+    (``ppvm_batch_run1_noalloc``).  Returns (x, z, c, overlap_last, term_updates)."""
+    x, z = pack_strings([s for s, _ in w.observable], w.n_qubits)
+    c = np.asarray([v for _, v in w.observable], dtype=np.float64)
+    prog = marked_program(w)
+    inst = BatchInstance()
+    inst.n_qubits = w.n_qubits
+    inst.n_terms = len(c)
+    inst.x = x.ctypes.data_as(_u64p)
+    inst.z = z.ctypes.data_as(_u64p)
+    inst.c = c.ctypes.data_as(_f64p)
+    inst.ops = prog.ctypes.data
+    inst.n_ops = len(prog)
+    inst.min_abs_coeff = w.min_abs_coeff
+    inst.max_pauli_weight = (1 << 64) - 1 if w.max_pauli_weight is None else w.max_pauli_weight
+    inst.capacity = capacity
+    W = words_for(w.n_qubits)
+    ox = np.zeros((out_cap, W), dtype=np.uint64)
+    oz = np.zeros((out_cap, W), dtype=np.uint64)
+    oc = np.zeros(out_cap, dtype=np.float64)
+    n = ctypes.c_size_t()
+    ovl = ctypes.c_double()
+    tu = ctypes.c_uint64()
+    _check(_lib().ppvm_batch_run1_noalloc(ctypes.byref(inst), threads, ox.ctypes.data_as(_u64p),
+                                          oz.ctypes.data_as(_u64p), oc.ctypes.data_as(_f64p), out_cap,
+                                          ctypes.byref(n), ctypes.byref(ovl), ctypes.byref(tu)))
+    k = n.value
+    return ox[:k], oz[:k], oc[:k], ovl.value, tu.value
 
 
 @dataclass

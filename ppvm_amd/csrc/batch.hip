@@ -32,6 +32,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -790,6 +791,43 @@ u64 dedupe_initial(const ppvm_batch_instance& s, u32 W, std::vector<u64>& kx, st
   return kc.size() - first;
 }
 
+// ---- allocation-free single-instance runs ----------------------------------------
+// Every device buffer of ppvm_batch_run1_noalloc (instance descriptor, program,
+// noise tables, arenas, partner list, hash table, queue) lives in this array:
+// kScratchBytes per GPU, held from the library's load until the process ends.
+// In host code `scratch` names a host-side stand-in; its device address comes
+// from hipGetSymbolAddress, once per device (scratch_on_device).
+constexpr size_t kScratchBytes = 64ull << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kScratchDevices] = {};
+
+// Device address of `scratch` on the current device (resolved on the first
+// call per device, with that device current).
+int scratch_on_device(unsigned char** out) {
+  int dev = 0;
+  BCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices) return bfail(PPVM_ENODEV, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return bfail(PPVM_EHIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return PPVM_OK;
+}
+
+__global__ void k_bfill_u64(u64* p, u64 n, u64 v) {
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (u64)gridDim.x * blockDim.x) p[i] = v;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
 }  // namespace
 
 extern "C" {
@@ -1065,6 +1103,115 @@ int ppvm_batch_export(ppvm_batch_handle b, size_t i, uint64_t* x, uint64_t* z, d
       x[t * wa + p] = pl[(size_t)p * I.n + t];
       z[t * wa + p] = pl[(size_t)(W + p) * I.n + t];
     }
+  return PPVM_OK;
+}
+
+int ppvm_batch_run1_noalloc(const ppvm_batch_instance* inst, uint32_t threads, uint64_t* x, uint64_t* z,
+                            double* c, size_t cap, size_t* n, double* overlap_last, uint64_t* term_updates) {
+  if (!inst || !n) return bfail(PPVM_EINVAL, "null argument");
+  int rc;
+  if ((rc = validate_instance(*inst, 0))) return rc;
+  if (threads == 0) threads = 256;
+  if (threads % 64 || threads > (u32)kBMaxThreads) return bfail(PPVM_EINVAL, "threads must be a multiple of 64 up to 1024");
+  unsigned char* base = nullptr;
+  if ((rc = scratch_on_device(&base))) return rc;
+  // host-side preparation (no device memory involved)
+  const u32 W = engine_words((inst->n_qubits + 63) / 64), np = 2 * W;
+  std::vector<BOp> ops;
+  std::vector<double> facs;
+  (void)translate_program(*inst, ops, facs);
+  std::vector<u64> kx, kz;
+  std::vector<double> kc;
+  const u64 n0 = dedupe_initial(*inst, W, kx, kz, kc);
+  const u64 acap = pow2_ge(std::max<u64>(inst->capacity, std::max<u64>(1024, 2 * n0)));
+  const u64 tcap = pow2_ge(2 * acap);
+  // scratch layout
+  size_t off = 0;
+  const size_t o_inst = off; off += align256(sizeof(BInst));
+  const size_t o_sup = off; off += align256(kSupWords * sizeof(u64));
+  const size_t o_ctl = off; off += 256;  // order (u32), queue (u32), t0 (u64)
+  const size_t o_ops = off; off += align256(std::max<size_t>(1, ops.size()) * sizeof(BOp));
+  const size_t o_fac = off; off += align256(std::max<size_t>(1, facs.size()) * sizeof(double));
+  const size_t o_pl = off; off += 2 * (size_t)np * acap * 8;
+  const size_t o_cf = off; off += 2 * (size_t)acap * 8;
+  const size_t o_pa = off; off += align256(acap * 4);
+  const size_t o_tb = off; off += tcap * 8;
+  if (off > kScratchBytes)
+    return bfail(PPVM_EINVAL, "run needs %zu bytes of scratch (arena of %llu terms, %zu ops); the maximum is %zu",
+                 off, (unsigned long long)acap, ops.size(), kScratchBytes);
+  BInst I{};
+  I.ops = reinterpret_cast<const BOp*>(base + o_ops);
+  I.factors = reinterpret_cast<const double*>(base + o_fac);
+  I.n_ops = inst->n_ops;
+  I.n_qubits = inst->n_qubits;
+  I.tau = inst->min_abs_coeff;
+  I.maxw = inst->max_pauli_weight;
+  I.cap = acap;
+  I.planes[0] = reinterpret_cast<u64*>(base + o_pl);
+  I.planes[1] = reinterpret_cast<u64*>(base + o_pl + (size_t)np * acap * 8);
+  I.coef[0] = reinterpret_cast<double*>(base + o_cf);
+  I.coef[1] = reinterpret_cast<double*>(base + o_cf + acap * 8);
+  I.partner = reinterpret_cast<u32*>(base + o_pa);
+  I.table = reinterpret_cast<u64*>(base + o_tb);
+  I.tcap = tcap;
+  I.n_marks = 0;  // marked ops still update the last overlap
+  I.n_init = n0;
+  u32* order = reinterpret_cast<u32*>(base + o_ctl);
+  u32* queue = order + 1;
+  u64* t0 = reinterpret_cast<u64*>(base + o_ctl + 8);
+  // uploads (the scratch array keeps the previous call's contents: everything
+  // read before it is written is set here)
+  std::vector<u64> pl((size_t)np * n0);
+  for (u64 t = 0; t < n0; ++t)
+    for (u32 p = 0; p < np; ++p) pl[(size_t)p * n0 + t] = (p < W ? kx : kz)[t * W + (p % W)];
+  const hipStream_t st = nullptr;  // the device's null stream
+  const u32 zero = 0;
+  BCHK(hipMemcpyAsync(base + o_inst, &I, sizeof(BInst), hipMemcpyHostToDevice, st));
+  BCHK(hipMemcpyAsync(order, &zero, sizeof(u32), hipMemcpyHostToDevice, st));
+  if (!ops.empty()) BCHK(hipMemcpyAsync(base + o_ops, ops.data(), ops.size() * sizeof(BOp), hipMemcpyHostToDevice, st));
+  if (!facs.empty()) BCHK(hipMemcpyAsync(base + o_fac, facs.data(), facs.size() * sizeof(double), hipMemcpyHostToDevice, st));
+  for (u32 p = 0; p < np && n0; ++p)
+    BCHK(hipMemcpyAsync(I.planes[0] + (size_t)p * acap, pl.data() + (size_t)p * n0, n0 * 8, hipMemcpyHostToDevice, st));
+  if (n0) BCHK(hipMemcpyAsync(I.coef[0], kc.data(), n0 * 8, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_bfill_u64, dim3((u32)std::min<u64>(1024, (tcap + 255) / 256)), dim3(256), 0, st, I.table, tcap, 0ull);
+  hipLaunchKernelGGL(k_bfill_u32, dim3(1), dim3(64), 0, st, queue, 1u, 0u);
+  BInst* d_inst = reinterpret_cast<BInst*>(base + o_inst);
+  u64* d_sup = reinterpret_cast<u64*>(base + o_sup);
+  hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, st, d_inst, 1u, t0);
+  const u32 lim = batch_gen_limit();
+  switch (W) {
+    case 1: hipLaunchKernelGGL(k_batch<1>, dim3(1), dim3(threads), 0, st, d_inst, d_sup, order, 1u, queue, 0u, inst->n_ops, lim); break;
+    case 2: hipLaunchKernelGGL(k_batch<2>, dim3(1), dim3(threads), 0, st, d_inst, d_sup, order, 1u, queue, 0u, inst->n_ops, lim); break;
+    case 4: hipLaunchKernelGGL(k_batch<4>, dim3(1), dim3(threads), 0, st, d_inst, d_sup, order, 1u, queue, 0u, inst->n_ops, lim); break;
+    default: hipLaunchKernelGGL(k_batch<8>, dim3(1), dim3(threads), 0, st, d_inst, d_sup, order, 1u, queue, 0u, inst->n_ops, lim); break;
+  }
+  BCHK(hipGetLastError());
+  BInst R{};
+  BCHK(hipMemcpyAsync(&R, d_inst, sizeof(BInst), hipMemcpyDeviceToHost, st));
+  BCHK(hipStreamSynchronize(st));
+  if (R.status == IS_OVERFLOW)
+    return bfail(PPVM_ENOMEM, "arena of %llu terms overflowed; an allocation-free run cannot grow it (pass a "
+                 "larger capacity, up to what %zu bytes of scratch hold)", (unsigned long long)acap, kScratchBytes);
+  if (R.status != IS_DONE) return bfail(PPVM_EHIP, "run did not finish (status %u)", R.status);
+  *n = R.n;
+  if (overlap_last) *overlap_last = R.ovl_last;
+  if (term_updates) *term_updates = R.term_updates;
+  if (cap < R.n) return bfail(PPVM_EINVAL, "export buffer holds %zu, %llu needed", cap, (unsigned long long)R.n);
+  if (R.n) {
+    if (!x || !z || !c) return bfail(PPVM_EINVAL, "null export array");
+    const u32 wa = (inst->n_qubits + 63) / 64;
+    std::vector<u64> out((size_t)np * R.n);
+    for (u32 p = 0; p < np; ++p)
+      BCHK(hipMemcpyAsync(out.data() + (size_t)p * R.n, R.planes[R.cur] + (size_t)p * acap, R.n * 8,
+                          hipMemcpyDeviceToHost, st));
+    BCHK(hipMemcpyAsync(c, R.coef[R.cur], R.n * 8, hipMemcpyDeviceToHost, st));
+    BCHK(hipStreamSynchronize(st));
+    for (u64 t = 0; t < R.n; ++t)
+      for (u32 p = 0; p < wa; ++p) {
+        x[t * wa + p] = out[(size_t)p * R.n + t];
+        z[t * wa + p] = out[(size_t)(W + p) * R.n + t];
+      }
+  }
   return PPVM_OK;
 }
 
