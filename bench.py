@@ -56,26 +56,27 @@ PAIRING_KERNELS = ("k_rot_pair", "k_pair_hist", "k_pair_colscan", "k_pair_scatte
 
 def phase_bytes(cnt: dict, W: int, nq: int) -> dict:
     """Algorithmic HBM bytes per kernel, summed over the run (S = 16W + 8 B per
-    term, R = 16W + 16 B per worklist record, N = scanned entries, N_a =
-    anticommuting terms, N_p = merged pairs, N_c = appended children, N_t =
-    entries transposed into the bit-sliced planes):
+    term: key words and coefficient; R = 16 B per worklist record: coefficient,
+    arena index, key facts; N = scanned entries, N_a = anticommuting terms,
+    N_p = merged pairs, N_c = appended children, N_t = entries transposed into
+    the bit-sliced planes):
 
     k_rot_count  : 1 bit per bit-sliced plane the generator needs for every
                    entry + N/8 B of ballot masks + N_t (16W + 2 nq / 8) for the
-                   entries tested from the key planes and transposed.
+                   entries tested from their keys and transposed.
     k_rot_gather : N/8 B masks + N_a (S + R + 8): key and coefficient read,
                    record and pair-key hash written.
-    pairing      : N_a 8 B hashes read + N_p (2 R + 32): both records of each
-                   merged pair read, each member's merged coefficient and flag
-                   written back into its record (16 B) (the minimum of either
-                   implementation; the partition's own traffic shows in the
-                   measured bytes).
+    pairing      : N_a 8 B hashes read + N_p 2 (R + 16W + 16): both members'
+                   records and arena keys read, both records rewritten (the
+                   minimum of either pairing implementation; the partition's
+                   own traffic shows in the measured bytes).
     k_emit_count : N_a R records read + N_a 8 B coefficients written (scaled
                    parents and merged pairs) + N_a/8 masks.
-    k_emit_write : N_a/8 masks + N_c (R + S): record read, child term written.
+    k_emit_write : N_a/8 masks + N_c (R + 16W + S): record and parent key read,
+                   child term written.
     """
     S = 16 * W + 8
-    R = 16 * W + 16
+    R = 16
     N = cnt["scanned"]
     na = cnt["wl_total"]
     npair = cnt["pairs_total"]
@@ -83,9 +84,9 @@ def phase_bytes(cnt: dict, W: int, nq: int) -> dict:
     return {
         "k_rot_count": cnt["scan_planes"] / 8 + N / 8 + cnt["transposed"] * (16 * W + 2 * nq / 8),
         "k_rot_gather": N / 8 + na * (S + R + 8),
-        "pairing": na * 8 + npair * (2 * R + 32),
+        "pairing": na * 8 + npair * 2 * (R + 16 * W + 16),
         "k_emit_count": na * R + na * 8 + na / 8,
-        "k_emit_write": na / 8 + nch * (R + S),
+        "k_emit_write": na / 8 + nch * (R + 16 * W + S),
     }
 
 

@@ -1,44 +1,44 @@
 // algebra.hpp — per-term Pauli algebra shared by the engine and batched kernels:
-// worklist records, bit access on register-resident keys, rotation branching
-// (anticommutation, child key, sign) and the map ops (Clifford / noise).
+// worklist records, key loads, bit access on register-resident keys, rotation
+// branching (anticommutation, child key, sign) and the map ops (Clifford / noise).
 #pragma once
 
 #include "device.hpp"
 
 namespace ppvm {
 
-template <int W>
-struct alignas(16) WLRec {
-  u64 k[2 * W];
+// Worklist record of one anticommuting arena entry (16 bytes): its coefficient,
+// arena index and the facts about its key that the emit and merge steps need,
+// taken by the gather while the key is in registers (so no later kernel reads
+// the key again except to confirm a candidate pair or to write a child).
+struct alignas(16) WMRec {
   double c;
   u32 idx;
-  u32 flags;  // 1 = merged with its partner by k_pair_lds, 2 = dead entry
+  u32 info;  // WM_* bits below
 };
+constexpr u32 WM_FLAGS = 3u;       // 0 unpaired, 1 merged with its partner, 2 dead entry
+constexpr u32 WM_EPS_NEG = 4u;     // the child's sign eps is -1
+constexpr u32 WM_ZA = 8u;          // z bit of the first generator qubit (damping: the Z member)
+constexpr u32 WM_P_OVER = 16u;     // the key's weight exceeds the cap
+constexpr u32 WM_P_PRES = 32u;     // the key is in the preserve set
+constexpr u32 WM_C_OVER = 64u;     // the child key's weight exceeds the cap
+constexpr u32 WM_C_PRES = 128u;    // the child key is in the preserve set
 
-// Worklist record load with 16-byte vector loads.
-template <int W>
-__device__ __forceinline__ void load_rec(const WLRec<W>* __restrict__ wl, u64 i, u64 (&k)[2 * W], double& c,
-                                         u32& idx, u32& flags) {
-  const ulonglong2* p = reinterpret_cast<const ulonglong2*>(wl + i);
-#pragma unroll
-  for (int q = 0; q < W; ++q) {
-    const ulonglong2 v = p[q];
-    k[2 * q] = v.x;
-    k[2 * q + 1] = v.y;
-  }
-  const ulonglong2 t = p[W];
-  c = __longlong_as_double((long long)t.x);
-  idx = (u32)t.y;
-  flags = (u32)(t.y >> 32);
+__device__ __forceinline__ void load_wm(const WMRec* __restrict__ wm, u64 r, double& c, u32& idx, u32& info) {
+  const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(wm + r);
+  c = __longlong_as_double((long long)v.x);
+  idx = (u32)v.y;
+  info = (u32)(v.y >> 32);
+}
+__device__ __forceinline__ void store_wm(WMRec* __restrict__ wm, u64 r, double c, u32 idx, u32 info) {
+  *reinterpret_cast<ulonglong2*>(wm + r) = make_ulonglong2((u64)__double_as_longlong(c), (u64)idx | ((u64)info << 32));
 }
 
-template <int W>
-__device__ __forceinline__ void store_rec(WLRec<W>* __restrict__ wl, u64 i, const u64 (&k)[2 * W], double c,
-                                          u32 idx, u32 flags) {
-  ulonglong2* p = reinterpret_cast<ulonglong2*>(wl + i);
-#pragma unroll
-  for (int q = 0; q < W; ++q) p[q] = make_ulonglong2(k[2 * q], k[2 * q + 1]);
-  p[W] = make_ulonglong2((u64)__double_as_longlong(c), (u64)idx | ((u64)flags << 32));
+// The truncation predicate drop_term (device.hpp) with the key's weight test
+// and preserve-set membership taken from the record: drop = (|c| < tau or
+// weight > cap) and not preserved.
+__device__ __forceinline__ bool drop_from_info(double c, bool over, bool pres, const TruncSpec& t) {
+  return (fabs(c) < t.tau || over) && !pres;
 }
 
 // Word access with a runtime index, without dynamic register indexing.
@@ -122,10 +122,32 @@ __device__ __forceinline__ void pair_key(const u64 (&k)[2 * W], const RotSpec& g
   if (piv) rot_flip<W>(r, g);
 }
 
+// Key of entry i from structure-of-arrays planes (plane p at p * cap); the
+// batched kernel's arenas.
 template <int W>
 __device__ __forceinline__ void load_key(const u64* __restrict__ planes, u64 cap, u64 i, u64 (&k)[2 * W]) {
 #pragma unroll
   for (int p = 0; p < 2 * W; ++p) k[p] = planes[(u64)p * cap + i];
+}
+
+// Key of entry i from an array-of-structures key arena (the single-sum
+// engine): the 2W words of an entry are contiguous, x words then z words, so
+// one entry is W 16-byte loads from one aligned 16W-byte block.
+template <int W>
+__device__ __forceinline__ void akey_load(const u64* __restrict__ keys, u64 i, u64 (&k)[2 * W]) {
+  const ulonglong2* p = reinterpret_cast<const ulonglong2*>(keys + i * (2 * W));
+#pragma unroll
+  for (int q = 0; q < W; ++q) {
+    const ulonglong2 v = p[q];
+    k[2 * q] = v.x;
+    k[2 * q + 1] = v.y;
+  }
+}
+template <int W>
+__device__ __forceinline__ void akey_store(u64* __restrict__ keys, u64 i, const u64 (&k)[2 * W]) {
+  ulonglong2* p = reinterpret_cast<ulonglong2*>(keys + i * (2 * W));
+#pragma unroll
+  for (int q = 0; q < W; ++q) p[q] = make_ulonglong2(k[2 * q], k[2 * q + 1]);
 }
 
 

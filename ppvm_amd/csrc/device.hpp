@@ -1,12 +1,14 @@
 // device.hpp — device-side building blocks of the gfx950 Pauli-propagation engine.
 //
 // Data layout in HBM (per PauliSum handle):
-//   planes : u64[2W][cap]   structure of arrays; plane p < W holds x word p of
-//                           every term, plane W+p holds z word p.
+//   keys   : u64[cap][2W]   array of structures: the 2W words of an entry are
+//                           contiguous (x words 0..W-1, then z words), so a
+//                           sparse gather reads one aligned 16W-byte block.
 //   coef   : f64[cap]       coefficient; a truncated ("dead") entry holds the
 //                           NaN pattern DEAD_BITS and is skipped by every kernel.
-//   wl     : WLRec<W>[cap]  per-gate worklist of anticommuting terms (AoS, one
-//                           record = full key + coefficient + arena index).
+//   wl     : WMRec[cap]     per-gate worklist of anticommuting terms, one
+//                           16-byte record each (coefficient, arena index,
+//                           facts about the key that later steps need).
 //   table  : u64[tcap]      open-addressed pair table (generation-tagged slots).
 //   masks  : u64[cap/64]    per-pass ballot bitmasks (bit j of word w = item 64w+j)
 //   parts  : u64[6][32768]  per-chunk counts and per-block partial sums.

@@ -116,11 +116,11 @@ struct ppvm_sum {
   bool trunc_active = false;
   int device = 0;
   hipStream_t stream = nullptr;
-  u64* planes[2] = {nullptr, nullptr};
+  u64* keys[2] = {nullptr, nullptr};  // arena keys: 2W words per entry, contiguous (x words, then z words)
   double* coef[2] = {nullptr, nullptr};
   int cur = 0;
   u64 cap = 0;
-  void* wl = nullptr;
+  WMRec* wl = nullptr;  // per-gate worklist: one 16-byte record per anticommuting entry
   u64* table = nullptr;
   u64 tcap = 0;
   u64* masks = nullptr;     // ballot masks, cap/64 + kMaxParts words
@@ -161,7 +161,6 @@ struct ppvm_sum {
   u64 prof_n[8] = {0};
   u64* part(int i) const { return parts + (size_t)i * kMaxParts; }
 
-  size_t wl_rec() const { return 16 * (size_t)W + 16; }
   TruncSpec trunc_spec() const {
     TruncSpec t;
     t.tau = tau;
@@ -224,9 +223,9 @@ u32 next_tab_gen(ppvm_sum* s) {
 // ---- allocation -------------------------------------------------------------------
 void free_buffers(ppvm_sum* s) {
   for (int i = 0; i < 2; ++i) {
-    if (s->planes[i]) (void)hipFree(s->planes[i]);
+    if (s->keys[i]) (void)hipFree(s->keys[i]);
     if (s->coef[i]) (void)hipFree(s->coef[i]);
-    s->planes[i] = nullptr;
+    s->keys[i] = nullptr;
     s->coef[i] = nullptr;
   }
   if (s->wl) (void)hipFree(s->wl);
@@ -270,19 +269,19 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
     (void)hipGetLastError();
     return fail(PPVM_ENOMEM, "device allocation of %llu-term arenas failed", (unsigned long long)new_cap);
   }
-  if (n_keep && s->planes[s->cur]) {
+  if (n_keep && s->keys[s->cur]) {
     hipLaunchKernelGGL(k_copy_arena, dim3(grid_for(n_keep, 256, 4096)), dim3(256), 0, s->stream,
-                       s->planes[s->cur], s->coef[s->cur], s->cap, np0, nc0, new_cap, np, n_keep);
+                       s->keys[s->cur], s->coef[s->cur], np0, nc0, np, n_keep);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(s->stream));
   for (int i = 0; i < 2; ++i) {
-    if (s->planes[i]) (void)hipFree(s->planes[i]);
+    if (s->keys[i]) (void)hipFree(s->keys[i]);
     if (s->coef[i]) (void)hipFree(s->coef[i]);
   }
-  s->planes[0] = np0;
+  s->keys[0] = np0;
   s->coef[0] = nc0;
-  s->planes[1] = np1;
+  s->keys[1] = np1;
   s->coef[1] = nc1;
   s->cur = 0;
   if (s->wl) (void)hipFree(s->wl);
@@ -299,7 +298,7 @@ int resize(ppvm_sum* s, u64 new_cap, u64 n_keep) {
   s->bkt = nullptr;
   s->tcap = pow2_at_least(2 * new_cap);
   s->tstride = new_cap / 64 + 8;  // multiple of 8 words; room for the last 8-word group
-  if (hipMalloc(&s->wl, new_cap * s->wl_rec()) != hipSuccess ||
+  if (hipMalloc(&s->wl, new_cap * sizeof(WMRec)) != hipSuccess ||
       hipMalloc(&s->table, s->tcap * 8) != hipSuccess ||
       hipMalloc(&s->masks, (new_cap / 64 + kMaxParts + 64) * 8) != hipSuccess ||
       hipMalloc(&s->tp, 2 * (size_t)s->n * s->tstride * 8) != hipSuccess ||
@@ -381,16 +380,16 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
   const u64 ne = grid_entries(s);
   const u32 G = std::min<u32>(chunk_blocks(ne, kArenaChunk), kArenaGrid);
   const u32 Gl = std::min<u32>(chunk_blocks(ne, kListChunk), kListGrid);
-  WLRec<W>* wl = reinterpret_cast<WLRec<W>*>(s->wl);
-  u64* planes = s->planes[s->cur];
+  WMRec* wl = s->wl;
+  u64* keys = s->keys[s->cur];
   double* coef = s->coef[s->cur];
   if ((rc = prof_begin(s, PH_COUNT))) return rc;
-  hipLaunchKernelGGL(k_rot_count<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, s->cap, s->tp, s->tstride,
+  hipLaunchKernelGGL(k_rot_count<W>, dim3(G), dim3(kBlock), 0, s->stream, keys, s->tp, s->tstride,
                      s->n, s->st, in, it.rot, s->masks, s->part(0));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_GATHER))) return rc;
-  hipLaunchKernelGGL(k_rot_gather<W>, dim3(G), dim3(kBlock), 0, s->stream, planes, coef, s->cap, s->st, in,
-                     s->masks, s->part(0), wl, it.rot, s->hsh);
+  hipLaunchKernelGGL(k_rot_gather<W>, dim3(G), dim3(kBlock), 0, s->stream, keys, coef, s->st, in,
+                     s->masks, s->part(0), wl, it.rot, s->trunc_spec(), s->hsh);
   if ((rc = prof_end(s))) return rc;
   const u32 tg = next_tab_gen(s);
   u32 Gp;
@@ -407,12 +406,12 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
                        s->cap, lgm, hist, bmeta);
     hipLaunchKernelGGL(k_pair_scatter, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st, in,
                        s->cap, lgm, hist, s->bkt, bmeta);
-    hipLaunchKernelGGL(k_pair_lds<W>, dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, wl, s->st, in,
+    hipLaunchKernelGGL(k_pair_lds<W>, dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, keys, wl, s->st, in,
                        it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(), s->part(1),
                        s->part(2));
   } else {
     Gp = (u32)grid_for(ne, kBlock * 2, kFixedGrid);
-    hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, s->cap, wl, s->st, in,
+    hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, s->cap, keys, wl, s->st, in,
                        it.rot, s->hsh, s->table, s->tcap, tg, s->trunc_spec(), it.seq + 1, s->part(1),
                        s->part(2));
   }
@@ -422,7 +421,7 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
                      it.rot, s->trunc_spec(), s->masks, s->part(3), s->part(4));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_EWRITE))) return rc;
-  hipLaunchKernelGGL(k_emit_write<W>, dim3(Gl), dim3(kBlock), 0, s->stream, planes, coef, s->cap, wl, s->st,
+  hipLaunchKernelGGL(k_emit_write<W>, dim3(Gl), dim3(kBlock), 0, s->stream, keys, coef, s->cap, wl, s->st,
                      in, out, it.rot, s->masks, s->part(3), s->part(4), s->part(1), s->part(2), Gp);
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
@@ -435,8 +434,8 @@ int launch_map(ppvm_sum* s, const Issued& it) {
   int rc;
   const u32 Gm = (u32)grid_for(grid_entries(s), kBlock * 4, kFixedGrid);
   if ((rc = prof_begin(s, PH_MAP))) return rc;
-  hipLaunchKernelGGL(k_map<W>, dim3(Gm), dim3(kBlock), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
-                     s->cap, s->st, in, it.map, s->trunc_spec(), s->part(0), s->part(1));
+  hipLaunchKernelGGL(k_map<W>, dim3(Gm), dim3(kBlock), 0, s->stream, s->keys[s->cur], s->coef[s->cur],
+                     s->st, in, it.map, s->trunc_spec(), s->part(0), s->part(1));
   if ((rc = prof_end(s))) return rc;
   FinishArgs fa{};
   fa.mode = F_MAP;
@@ -460,8 +459,8 @@ int launch_overlap(ppvm_sum* s) {
   int rc;
   const u32 in = (u32)(s->op_seq & 1);
   if ((rc = prof_begin(s, PH_OTHER))) return rc;
-  hipLaunchKernelGGL(k_overlap<W>, dim3(kOverlapGrid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
-                     s->coef[s->cur], s->cap, s->st, in, s->partials);
+  hipLaunchKernelGGL(k_overlap<W>, dim3(kOverlapGrid), dim3(kBlock), 0, s->stream, s->keys[s->cur],
+                     s->coef[s->cur], s->st, in, s->partials);
   if ((rc = prof_end(s))) return rc;
   FinishArgs fa{};
   fa.mode = F_OVERLAP;
@@ -477,8 +476,8 @@ int launch_trace(ppvm_sum* s, const PatternProg& prog) {
   int rc;
   const u32 in = (u32)(s->op_seq & 1);
   if ((rc = prof_begin(s, PH_OTHER))) return rc;
-  hipLaunchKernelGGL(k_trace<W>, dim3(kOverlapGrid), dim3(kBlock), 0, s->stream, s->planes[s->cur],
-                     s->coef[s->cur], s->cap, s->st, in, prog, s->partials);
+  hipLaunchKernelGGL(k_trace<W>, dim3(kOverlapGrid), dim3(kBlock), 0, s->stream, s->keys[s->cur],
+                     s->coef[s->cur], s->st, in, prog, s->partials);
   if ((rc = prof_end(s))) return rc;
   FinishArgs fa{};
   fa.mode = F_OVERLAP;
@@ -499,11 +498,11 @@ int launch_overlap2(ppvm_sum* a, ppvm_sum* b) {
   const u64 tmask = a->tcap - 1;
   if ((rc = prof_begin(a, PH_OTHER))) return rc;
   hipLaunchKernelGGL(k_ovl_build<W>, dim3(grid_for(a->n_bound, kBlock * 4, kFixedGrid)), dim3(kBlock), 0,
-                     a->stream, a->planes[a->cur], a->coef[a->cur], a->cap, a->st, ia, a->table, tmask, tg);
+                     a->stream, a->keys[a->cur], a->coef[a->cur], a->st, ia, a->table, tmask, tg);
   if ((rc = prof_end(a))) return rc;
   if ((rc = prof_begin(a, PH_OTHER))) return rc;
-  hipLaunchKernelGGL(k_ovl_probe<W>, dim3(kOverlapGrid), dim3(kBlock), 0, a->stream, a->planes[a->cur],
-                     a->coef[a->cur], a->cap, a->table, tmask, tg, b->planes[b->cur], b->coef[b->cur], b->cap,
+  hipLaunchKernelGGL(k_ovl_probe<W>, dim3(kOverlapGrid), dim3(kBlock), 0, a->stream, a->keys[a->cur],
+                     a->coef[a->cur], a->table, tmask, tg, b->keys[b->cur], b->coef[b->cur],
                      b->st, ib, a->partials);
   if ((rc = prof_end(a))) return rc;
   FinishArgs fa{};
@@ -525,8 +524,8 @@ int launch_compact(ppvm_sum* s) {
                      s->masks, s->part(0));
   if ((rc = prof_end(s))) return rc;
   if ((rc = prof_begin(s, PH_OTHER))) return rc;
-  hipLaunchKernelGGL(k_compact_write<W>, dim3(G), dim3(kBlock), 0, s->stream, s->planes[s->cur],
-                     s->coef[s->cur], s->planes[s->cur ^ 1], s->coef[s->cur ^ 1], s->cap, s->st, in, out,
+  hipLaunchKernelGGL(k_compact_write<W>, dim3(G), dim3(kBlock), 0, s->stream, s->keys[s->cur],
+                     s->coef[s->cur], s->keys[s->cur ^ 1], s->coef[s->cur ^ 1], s->st, in, out,
                      s->masks, s->part(0));
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
@@ -674,8 +673,8 @@ constexpr u64 kSupportRefreshMax = 16u << 20;
 
 template <int W>
 int launch_support(ppvm_sum* s, u32 G) {
-  hipLaunchKernelGGL(k_support<W>, dim3(G), dim3(kBlock), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
-                     s->cap, s->st, (u32)(s->op_seq & 1), s->part(5));
+  hipLaunchKernelGGL(k_support<W>, dim3(G), dim3(kBlock), 0, s->stream, s->keys[s->cur], s->coef[s->cur],
+                     s->st, (u32)(s->op_seq & 1), s->part(5));
   HIPCHK(hipGetLastError());
   return PPVM_OK;
 }
@@ -709,8 +708,8 @@ int refresh_support(ppvm_sum* s, u64 n) {
 
 template <int W>
 int launch_fingerprint(ppvm_sum* s, u32 G) {
-  hipLaunchKernelGGL(k_fingerprint<W>, dim3(G), dim3(kBlock), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
-                     s->cap, s->st, (u32)(s->op_seq & 1), s->Wa, s->part(5));
+  hipLaunchKernelGGL(k_fingerprint<W>, dim3(G), dim3(kBlock), 0, s->stream, s->keys[s->cur], s->coef[s->cur],
+                     s->st, (u32)(s->op_seq & 1), s->Wa, s->part(5));
   HIPCHK(hipGetLastError());
   return PPVM_OK;
 }
@@ -1156,12 +1155,10 @@ int download_live(ppvm_sum* s, std::vector<u64>& keys, std::vector<double>& cs) 
   const CountSlot c = s->st_host->slot[s->op_seq & 1];
   const u64 n = c.n_entries;
   const u32 np = 2 * s->W;
-  std::vector<u64> pl(n * np);
+  std::vector<u64> pl(n * np);  // term-major, as the arena holds them
   std::vector<double> cf(n);
   if (n) {
-    for (u32 p = 0; p < np; ++p)
-      HIPCHK(hipMemcpyAsync(pl.data() + (size_t)p * n, s->planes[s->cur] + (size_t)p * s->cap, n * 8,
-                            hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(pl.data(), s->keys[s->cur], n * np * 8, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipMemcpyAsync(cf.data(), s->coef[s->cur], n * 8, hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
   }
@@ -1173,7 +1170,7 @@ int download_live(ppvm_sum* s, std::vector<u64>& keys, std::vector<double>& cs) 
     u64 bits;
     std::memcpy(&bits, &cf[i], 8);
     if ((bits & ABS_MASK) == DEAD_BITS) continue;
-    for (u32 p = 0; p < np; ++p) keys.push_back(pl[(size_t)p * n + i]);
+    keys.insert(keys.end(), pl.begin() + (size_t)i * np, pl.begin() + (size_t)(i + 1) * np);
     cs.push_back(cf[i]);
   }
   return PPVM_OK;
@@ -1187,13 +1184,8 @@ int upload(ppvm_sum* s, const std::vector<u64>& keys, const std::vector<double>&
   if (n + 2 * n > s->cap) {
     if ((rc = resize(s, pow2_at_least(std::max<u64>(kMinCap, 4 * n)), 0))) return rc;
   }
-  std::vector<u64> pl((size_t)n * np);
-  for (u64 i = 0; i < n; ++i)
-    for (u32 p = 0; p < np; ++p) pl[(size_t)p * n + i] = keys[i * np + p];
-  if (n) {
-    for (u32 p = 0; p < np; ++p)
-      HIPCHK(hipMemcpyAsync(s->planes[s->cur] + (size_t)p * s->cap, pl.data() + (size_t)p * n, n * 8,
-                            hipMemcpyHostToDevice, s->stream));
+  if (n) {  // term-major keys are the arena layout
+    HIPCHK(hipMemcpyAsync(s->keys[s->cur], keys.data(), (size_t)n * np * 8, hipMemcpyHostToDevice, s->stream));
     HIPCHK(hipMemcpyAsync(s->coef[s->cur], cs.data(), n * 8, hipMemcpyHostToDevice, s->stream));
   }
   CountSlot c{};
@@ -1358,8 +1350,7 @@ int ppvm_clone(ppvm_handle h, ppvm_handle* out) {
   const u32 np = 2 * h->W;
   if (cs.n_entries) {
     hipLaunchKernelGGL(k_copy_arena, dim3(grid_for(cs.n_entries, 256, 4096)), dim3(256), 0, h->stream,
-                       h->planes[h->cur], h->coef[h->cur], h->cap, c->planes[c->cur], c->coef[c->cur], c->cap,
-                       np, cs.n_entries);
+                       h->keys[h->cur], h->coef[h->cur], c->keys[c->cur], c->coef[c->cur], np, cs.n_entries);
     HIPCHK(hipGetLastError());
   }
   HIPCHK(hipStreamSynchronize(h->stream));

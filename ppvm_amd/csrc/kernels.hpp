@@ -3,9 +3,10 @@
 // One rotation gate exp(-i theta/2 G) runs as:
 //   k_rot_count  : stream the test word(s) of every entry; ballot masks of the
 //                  anticommuting entries + one count per block chunk.
-//   k_rot_gather : copy each anticommuting entry (full key, coefficient, arena
-//                  index) into the worklist, in arena order (dead entries are
-//                  flagged, not copied out).
+//   k_rot_gather : one 16-byte record per anticommuting entry (coefficient,
+//                  arena index, the child's sign and the truncation tests of
+//                  the key and its child) plus its pair-key hash, in arena
+//                  order (dead entries are flagged).
 //   k_pair_hist, k_pair_scatter, k_pair_lds : radix-partition the worklist
 //                  on the hash of the pair key (key with the generator's pivot
 //                  bit cleared) and pair each bucket in an LDS hash table; the
@@ -18,7 +19,8 @@
 //                  survives the truncation; merged records store their new
 //                  coefficient in the arena; masks + counts.
 //   k_emit_write : append the surviving children at the arena end, in
-//                  worklist order; publish the new counts and statistics.
+//                  worklist order (parent key read from the arena, generator
+//                  bits flipped); publish the new counts and statistics.
 // Commuting entries are never touched: their only traffic is the test word.
 //
 // Floating point: products and sums use __dmul_rn / __dadd_rn (no contraction),
@@ -218,7 +220,7 @@ __device__ __forceinline__ u64 transpose64(u64 x, u32 lane) {
 // records are flagged by k_rot_gather, which reads every candidate's
 // coefficient anyway).
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ planes, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ keys,
                                                       u64* __restrict__ tp, u64 tstride, u32 nq,
                                                       const DevState* __restrict__ st, u32 in_slot,
                                                       RotSpec g, u64* __restrict__ masks,
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ pl
       u64 k[2 * W];
 #pragma unroll
       for (int p = 0; p < 2 * W; ++p) k[p] = 0;
-      if (in) load_key<W>(planes, cap, i, k);
+      if (in) akey_load<W>(keys, i, k);
       const u64 m = __ballot(in && rot_anti<W>(k, g) != 0u);
       if (lane == 0) {
         masks[w] = m;
@@ -290,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ pl
 #pragma unroll
       for (int j = 0; j < 8; ++j) {
         const u64 i = (wb + (u64)j) * 64 + lane;
-        r[j] = i < n ? planes[(u64)p * cap + i] : 0ull;
+        r[j] = i < n ? keys[i * (2 * W) + (u64)p] : 0ull;
       }
 #pragma unroll
       for (int j = 0; j < 8; ++j) r[j] = transpose64(r[j], lane);
@@ -311,18 +313,41 @@ __device__ __forceinline__ u64 pair_hash(u64 h, bool dead) {
   return dead ? kDeadHash : (h == kDeadHash ? kDeadHash - 1ull : h);
 }
 
+// Facts about an anticommuting key that the emit and merge steps need (the
+// WM_* bits of its worklist record): the child's sign, the damping member, and
+// the weight-cap and preserve-set tests of the key and of its child.
+template <int W>
+__device__ __forceinline__ u32 rec_info(const u64 (&k)[2 * W], const RotSpec& g, const TruncSpec& t) {
+  u32 info = 0u;
+  if (rot_eps<W>(k, g) < 0.0) info |= WM_EPS_NEG;
+  if (zbit<W>(k, g.a)) info |= WM_ZA;
+  const bool capped = t.maxw != ~0ull;
+  if (capped && weight_of<W>(k) > t.maxw) info |= WM_P_OVER;
+  if (t.n_preserve && is_preserved<W>(k, t)) info |= WM_P_PRES;
+  if (capped || t.n_preserve) {
+    u64 ch[2 * W];
+#pragma unroll
+    for (int p = 0; p < 2 * W; ++p) ch[p] = k[p];
+    rot_flip<W>(ch, g);
+    if (capped && weight_of<W>(ch) > t.maxw) info |= WM_C_OVER;
+    if (t.n_preserve && is_preserved<W>(ch, t)) info |= WM_C_PRES;
+  }
+  return info;
+}
+
 // ---- k_rot_gather ---------------------------------------------------------------
-// Copies every anticommuting entry into the worklist in arena order, and the
-// hash of its pair key into hsh; records of dead entries are kept (flags = 2,
-// hash kDeadHash) so the ranks match the masks.  Block 0 also publishes the
+// Writes one 16-byte record (coefficient, arena index, facts about the key)
+// per anticommuting entry into the worklist in arena order, and the hash of
+// its pair key into hsh; records of dead entries are kept (flags 2, hash
+// kDeadHash) so the ranks match the masks.  Block 0 also publishes the
 // worklist length.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ planes,
-                                                       const double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ keys,
+                                                       const double* __restrict__ coef,
                                                        DevState* st, u32 in_slot,
                                                        const u64* __restrict__ masks,
                                                        const u64* __restrict__ cnt,
-                                                       WLRec<W>* __restrict__ wl, RotSpec g,
+                                                       WMRec* __restrict__ wm, RotSpec g, TruncSpec t,
                                                        u64* __restrict__ hsh) {
   __shared__ u64 lds[kWaves];
   __shared__ u64 lm[kChunkMaxWords];
@@ -355,10 +380,10 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ p
     for (u32 r = threadIdx.x; r < total; r += kBlock) {
       const u64 i = lo + chunk_rank_pos(lm, loff, nw, r);
       u64 k[2 * W];
-      load_key<W>(planes, cap, i, k);
+      akey_load<W>(keys, i, k);
       const double c = coef[i];
       const bool dead = is_dead(c);
-      store_rec<W>(wl, base_off + r, k, c, (u32)i, dead ? 2u : 0u);
+      store_wm(wm, base_off + r, c, (u32)i, dead ? 2u : rec_info<W>(k, g, t));
       u64 rep[2 * W];
       pair_key<W>(k, g, rep);
       hsh[base_off + r] = pair_hash(hash_key<W>(rep), dead);
@@ -419,21 +444,23 @@ __device__ __forceinline__ u32 block_excl_scan_u32(u32 v, u32* wsum) {
   return pre + incl - v;
 }
 
-// Records i, j whose fingerprints agree: if their pair keys agree too, both
-// merged coefficients
+// Records i, j whose fingerprints agree: if their pair keys (read from the
+// arena through the records' indices) agree too, both merged coefficients
 //   c_i' = c_i cos + eps_j c_j sin,  c_j' = c_j cos + eps_i c_i sin
 // (damping: the Z member keeps c_Z cos, the I member receives c_I + c_Z sin),
 // after fused truncation, go into the records themselves with the merged flag
 // (one 16-byte store each); k_emit_count, which streams the worklist in arena
 // order anyway, moves them into the arena.
 template <int W>
-__device__ __forceinline__ bool pair_try_merge(WLRec<W>* __restrict__ wl, u64 i,
+__device__ __forceinline__ bool pair_try_merge(const u64* __restrict__ keys, WMRec* __restrict__ wm, u64 i,
                                                u64 j, const RotSpec& g, const TruncSpec& t, u64& kills) {
-  u64 k[2 * W], kj[2 * W];
   double ci, cj;
-  u32 idxi, idxj, fli, flj;
-  load_rec<W>(wl, i, k, ci, idxi, fli);
-  load_rec<W>(wl, j, kj, cj, idxj, flj);
+  u32 idxi, idxj, fi, fj;
+  load_wm(wm, i, ci, idxi, fi);
+  load_wm(wm, j, cj, idxj, fj);
+  u64 k[2 * W], kj[2 * W];
+  akey_load<W>(keys, idxi, k);
+  akey_load<W>(keys, idxj, kj);
   u64 rep[2 * W], repj[2 * W];
   pair_key<W>(k, g, rep);
   pair_key<W>(kj, g, repj);
@@ -443,24 +470,23 @@ __device__ __forceinline__ bool pair_try_merge(WLRec<W>* __restrict__ wl, u64 i,
   if (!same) return false;
   double ni, nj;
   if (g.mode == kBranchDamping) {
-    const bool zi = zbit<W>(k, g.a) != 0u;
+    const bool zi = (fi & WM_ZA) != 0u;
     const double cz = zi ? ci : cj, cI = zi ? cj : ci;
     const double nz = __dmul_rn(cz, g.cs), nI = __dadd_rn(cI, __dmul_rn(cz, g.sn));
     ni = zi ? nz : nI;
     nj = zi ? nI : nz;
   } else {
-    const double ei = rot_eps<W>(k, g), ej = rot_eps<W>(kj, g);
+    const double ei = (fi & WM_EPS_NEG) ? -1.0 : 1.0, ej = (fj & WM_EPS_NEG) ? -1.0 : 1.0;
     ni = __dadd_rn(__dmul_rn(ci, g.cs), __dmul_rn(cj, ej * g.sn));
     nj = __dadd_rn(__dmul_rn(cj, g.cs), __dmul_rn(ci, ei * g.sn));
   }
   bool di = false, dj = false;
   if (g.trunc) {
-    di = drop_term<W>(k, ni, t);
-    dj = drop_term<W>(kj, nj, t);
+    di = drop_from_info(ni, (fi & WM_P_OVER) != 0u, (fi & WM_P_PRES) != 0u, t);
+    dj = drop_from_info(nj, (fj & WM_P_OVER) != 0u, (fj & WM_P_PRES) != 0u, t);
   }
-  const double vi = di ? dead_value() : ni, vj = dj ? dead_value() : nj;
-  reinterpret_cast<ulonglong2*>(wl + i)[W] = make_ulonglong2((u64)__double_as_longlong(vi), (u64)idxi | (1ull << 32));
-  reinterpret_cast<ulonglong2*>(wl + j)[W] = make_ulonglong2((u64)__double_as_longlong(vj), (u64)idxj | (1ull << 32));
+  store_wm(wm, i, di ? dead_value() : ni, idxi, (fi & ~WM_FLAGS) | 1u);
+  store_wm(wm, j, dj ? dead_value() : nj, idxj, (fj & ~WM_FLAGS) | 1u);
   kills += (u64)di + (u64)dj;
   return true;
 }
@@ -469,7 +495,7 @@ __device__ __forceinline__ bool pair_try_merge(WLRec<W>* __restrict__ wl, u64 i,
 // an empty slot (first member of its pair so far) or merges with the member
 // already there.  Returns 1 if it merged.
 template <int W>
-__device__ __forceinline__ u64 pair_global(WLRec<W>* __restrict__ wl, u64 i, u64 h,
+__device__ __forceinline__ u64 pair_global(const u64* __restrict__ keys, WMRec* __restrict__ wm, u64 i, u64 h,
                                            u64* table, u64 tmask, u32 tgen, const RotSpec& g,
                                            const TruncSpec& t, u64& kills) {
   const u64 hg = mix64(h ^ 0x5bd1e9955bd1e995ull);
@@ -486,7 +512,7 @@ __device__ __forceinline__ u64 pair_global(WLRec<W>* __restrict__ wl, u64 i, u64
       sv = old;
       continue;
     }
-    if (((sv >> 32) & 0xffull) == fp && pair_try_merge<W>(wl, i, sv & 0xffffffffull, g, t, kills))
+    if (((sv >> 32) & 0xffull) == fp && pair_try_merge<W>(keys, wm, i, sv & 0xffffffffull, g, t, kills))
       return 1;
     pos = (pos + 1) & tmask;
     sv = table[pos];
@@ -502,8 +528,8 @@ __device__ __forceinline__ u64 pair_table_mask(u64 n_wl, u64 tcap_max) {
 // ---- k_rot_pair -----------------------------------------------------------------
 // Global-table pairing, one record per thread (grid-stride).
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_rot_pair(u64 cap,
-                                                     WLRec<W>* __restrict__ wl, DevState* st,
+__global__ __launch_bounds__(kBlock) void k_rot_pair(u64 cap, const u64* __restrict__ keys,
+                                                     WMRec* __restrict__ wm, DevState* st,
                                                      u32 in_slot, RotSpec g, const u64* __restrict__ hsh,
                                                      u64* table, u64 tcap_max, u32 tgen, TruncSpec t,
                                                      u64 op_tag, u64* __restrict__ part_kills,
@@ -521,7 +547,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(u64 cap,
   const u64 stride = (u64)gridDim.x * kBlock;
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n_wl; i += stride) {
     const u64 h = hsh[i];
-    if (h != kDeadHash) pairs += pair_global<W>(wl, i, h, table, tmask, tgen, g, t, kills);
+    if (h != kDeadHash) pairs += pair_global<W>(keys, wm, i, h, table, tmask, tgen, g, t, kills);
   }
   const u64 bk = block_sum_u64(kills, lds_sum);
   const u64 bp = block_sum_u64(pairs, lds_sum);
@@ -665,7 +691,7 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_scatter(const u64* __rest
 // too large, or a bucket whose positions exceed 20 bits, uses the global
 // table.  Pair members always share bucket and sub-bin.
 template <int W>
-__device__ __forceinline__ u64 lds_probe_sync(WLRec<W>* __restrict__ wl,
+__device__ __forceinline__ u64 lds_probe_sync(const u64* __restrict__ keys, WMRec* __restrict__ wm,
                                               u32* tab, const u64* __restrict__ bkt, u32 base, u32 e, u64 v,
                                               const RotSpec& g, const TruncSpec& t, u64& kills) {
   const u32 lo = (u32)(v >> 32);
@@ -684,15 +710,15 @@ __device__ __forceinline__ u64 lds_probe_sync(WLRec<W>* __restrict__ wl,
     }
     if ((sv >> 20) == fp && (sv & kPosLimit) != e + 1u) {
       const u32 j = (u32)bkt[base + (sv & kPosLimit) - 1u];
-      if (pair_try_merge<W>(wl, i, j, g, t, kills)) return 1;
+      if (pair_try_merge<W>(keys, wm, i, j, g, t, kills)) return 1;
     }
     pos = (pos + 1u) & (kLdsSlots - 1u);
   }
 }
 
 template <int W>
-__global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap,
-                                                           WLRec<W>* __restrict__ wl, DevState* st,
+__global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap, const u64* __restrict__ keys,
+                                                           WMRec* __restrict__ wm, DevState* st,
                                                            u32 in_slot, RotSpec g, u32 lg_max,
                                                            const u64* __restrict__ hsh,
                                                            const u64* __restrict__ bkt,
@@ -761,7 +787,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap,
       for (u32 c = tid; c < nc; c += kPairThreads) {
         const u32 ce = cand[c] & 0xffffu, cj = cand[c] >> 16;
         const u32 i = (u32)bkt[base + ce], j = (u32)bkt[base + cj];
-        if (pair_try_merge<W>(wl, i, j, g, t, kills)) {
+        if (pair_try_merge<W>(keys, wm, i, j, g, t, kills)) {
           ++pairs;
         } else {
           cand[c] |= 0x80000000u;  // fingerprint collision: walk again below
@@ -772,7 +798,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap,
         for (u32 c = tid; c < nc; c += kPairThreads) {
           if (!(cand[c] & 0x80000000u)) continue;
           const u32 ce = cand[c] & 0xffffu;
-          pairs += lds_probe_sync<W>(wl, tab, bkt, base, ce, bkt[base + ce], g, t, kills);
+          pairs += lds_probe_sync<W>(keys, wm, tab, bkt, base, ce, bkt[base + ce], g, t, kills);
         }
       }
       __syncthreads();  // table and candidate list are reset for the next bucket
@@ -821,7 +847,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap,
         for (u32 e = tid; e < T; e += kPairThreads) {
           const u64 v = bkt[base + e];
           const u32 sb = (u32)(v >> 58);
-          if (sb >= s0 && sb < s1) pairs += lds_probe_sync<W>(wl, tab, bkt, base, e, v, g, t, kills);
+          if (sb >= s0 && sb < s1) pairs += lds_probe_sync<W>(keys, wm, tab, bkt, base, e, v, g, t, kills);
         }
         __syncthreads();  // the table is cleared again by the next pass
       } else {
@@ -830,7 +856,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap,
           const u32 sb = (u32)(v >> 58);
           if (sb >= s0 && sb < s1) {
             const u32 i = (u32)v;
-            pairs += pair_global<W>(wl, i, hsh[i], table, tmask, tgen, g, t, kills);
+            pairs += pair_global<W>(keys, wm, i, hsh[i], table, tmask, tgen, g, t, kills);
             ++fallback;
           }
         }
@@ -863,9 +889,13 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap,
 }
 
 // ---- k_emit_count -----------------------------------------------------------------
+// Streams the 16-byte records: unpaired records scale their parent in place
+// and decide whether their child survives the fused truncation (from the
+// record's key facts, without the key); merged records move their final value
+// into the arena; masks + counts of the surviving children.
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef, u64 cap,
-                                                       const WLRec<W>* __restrict__ wl,
+                                                       const WMRec* __restrict__ wm,
                                                        const DevState* __restrict__ st, u32 in_slot,
                                                        RotSpec g, TruncSpec t, u64* __restrict__ masks,
                                                        u64* __restrict__ cnt, u64* __restrict__ kills_out) {
@@ -884,23 +914,22 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
     for (u32 j = wave; j < nw; j += kWaves) {
       const u64 w = lo + (u64)j * 64 + lane;
       bool keep = false;
-      u64 k[2 * W];
       double c = 0.0;
-      u32 idx = 0, fl = 2u;  // lanes past the chunk end act as dead records
-      if (w < hi) load_rec<W>(wl, w, k, c, idx, fl);
+      u32 idx = 0, info = 2u;  // lanes past the chunk end act as dead records
+      if (w < hi) load_wm(wm, w, c, idx, info);
+      const u32 fl = info & WM_FLAGS;
       if (fl == 0u) {
         // unpaired: the parent is scaled in place and its child survives
         // unless truncated (damping: only Z members branch, I members stay)
-        const bool branches = g.mode != kBranchDamping || zbit<W>(k, g.a) != 0u;
-        const double eps = rot_eps<W>(k, g);
+        const bool branches = g.mode != kBranchDamping || (info & WM_ZA) != 0u;
+        const double eps = (info & WM_EPS_NEG) ? -1.0 : 1.0;
         const double cp = branches ? __dmul_rn(c, g.cs) : c;
         const double cc = __dmul_rn(c, eps * g.sn);
         bool dp = false;
         keep = branches;
         if (g.trunc) {
-          dp = drop_term<W>(k, cp, t);
-          rot_flip<W>(k, g);
-          keep = branches && !drop_term<W>(k, cc, t);
+          dp = drop_from_info(cp, (info & WM_P_OVER) != 0u, (info & WM_P_PRES) != 0u, t);
+          keep = branches && !drop_from_info(cc, (info & WM_C_OVER) != 0u, (info & WM_C_PRES) != 0u, t);
         }
         if (dp || branches) coef[idx] = dp ? dead_value() : cp;
         kills += (u64)dp;
@@ -921,11 +950,12 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
 }
 
 // ---- k_emit_write -----------------------------------------------------------------
-// Appends the surviving children; block 0 also publishes the op's new counts
-// and statistics.
+// Appends the surviving children (the parent's key read from the arena, the
+// generator's bits flipped); block 0 also publishes the op's new counts and
+// statistics.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes, double* __restrict__ coef,
-                                                       u64 cap, const WLRec<W>* __restrict__ wl,
+__global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ keys, double* __restrict__ coef,
+                                                       u64 cap, const WMRec* __restrict__ wm,
                                                        DevState* st, u32 in_slot, u32 out_slot,
                                                        RotSpec g, const u64* __restrict__ masks,
                                                        const u64* __restrict__ cnt,
@@ -971,16 +1001,16 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
     const u32 total = nw ? loff[nw - 1] + (u32)__popcll(lm[nw - 1]) : 0u;
     for (u32 r = threadIdx.x; r < total; r += kBlock) {
       const u64 w = lo + chunk_rank_pos(lm, loff, nw, r);
-      u64 k[2 * W];
       double c;
-      u32 idx, fl;
-      load_rec<W>(wl, w, k, c, idx, fl);
-      const double eps = rot_eps<W>(k, g);
+      u32 idx, info;
+      load_wm(wm, w, c, idx, info);
+      u64 k[2 * W];
+      akey_load<W>(keys, idx, k);  // the parent (below n_entries; children go above)
+      const double eps = (info & WM_EPS_NEG) ? -1.0 : 1.0;
       const double cc = __dmul_rn(c, eps * g.sn);
       rot_flip<W>(k, g);
       const u64 dst = base_off + r;
-#pragma unroll
-      for (int p = 0; p < 2 * W; ++p) planes[(u64)p * cap + dst] = k[p];
+      akey_store<W>(keys, dst, k);
       coef[dst] = cc;
     }
     __syncthreads();  // lm / loff are reused by the next chunk
@@ -992,8 +1022,8 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ planes,
 // out[block * 2W + p]; the host ORs the blocks.  Lets the host judge exactly
 // which rotations can act on some term (see k_skip_gates).
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_support(const u64* __restrict__ planes,
-                                                    const double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_support(const u64* __restrict__ keys,
+                                                    const double* __restrict__ coef,
                                                     const DevState* __restrict__ st, u32 in_slot,
                                                     u64* __restrict__ out) {
   __shared__ u64 lds[2 * W][kWaves];
@@ -1005,7 +1035,7 @@ __global__ __launch_bounds__(kBlock) void k_support(const u64* __restrict__ plan
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
     if (is_dead(coef[i])) continue;
 #pragma unroll
-    for (int p = 0; p < 2 * W; ++p) acc[p] |= planes[(u64)p * cap + i];
+    for (int p = 0; p < 2 * W; ++p) acc[p] |= keys[i * (2 * W) + (u64)p];
   }
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
 #pragma unroll
@@ -1032,8 +1062,8 @@ __device__ __forceinline__ u64 fp_fmix(u64 v) {
   return v ^ (v >> 31);
 }
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_fingerprint(const u64* __restrict__ planes,
-                                                        const double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_fingerprint(const u64* __restrict__ keys,
+                                                        const double* __restrict__ coef,
                                                         const DevState* __restrict__ st, u32 in_slot, u32 wa,
                                                         u64* __restrict__ out) {
   __shared__ u64 lds[kWaves];
@@ -1044,8 +1074,8 @@ __global__ __launch_bounds__(kBlock) void k_fingerprint(const u64* __restrict__ 
     const double c = coef[i];
     if (is_dead(c)) continue;
     u64 v = 0x9E3779B97F4A7C15ull;
-    for (u32 p = 0; p < wa; ++p) v = fp_fmix(v ^ planes[(u64)p * cap + i]);
-    for (u32 p = 0; p < wa; ++p) v = fp_fmix(v ^ planes[(u64)(W + p) * cap + i]);
+    for (u32 p = 0; p < wa; ++p) v = fp_fmix(v ^ keys[i * (2 * W) + p]);
+    for (u32 p = 0; p < wa; ++p) v = fp_fmix(v ^ keys[i * (2 * W) + W + p]);
     ks += v;
     kcs += fp_fmix(v ^ (u64)__double_as_longlong(c));
   }
@@ -1137,8 +1167,8 @@ __global__ __launch_bounds__(kScanThreads) void k_finish(DevState* st, FinishArg
 
 // ---- k_map: Clifford / noise / truncation program over every entry ------------
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ planes,
-                                                double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ keys,
+                                                double* __restrict__ coef,
                                                 const DevState* __restrict__ st, u32 in_slot,
                                                 MapProgram prog, TruncSpec t,
                                                 u64* __restrict__ part_kills,
@@ -1152,7 +1182,7 @@ __global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ planes,
     double c = coef[i];
     if (is_dead(c)) continue;
     u64 k[2 * W], k0[2 * W];
-    load_key<W>(planes, cap, i, k);
+    akey_load<W>(keys, i, k);
 #pragma unroll
     for (int p = 0; p < 2 * W; ++p) k0[p] = k[p];
     const double c0 = c;
@@ -1170,7 +1200,7 @@ __global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ planes,
     }
 #pragma unroll
     for (int p = 0; p < 2 * W; ++p)
-      if (k[p] != k0[p]) planes[(u64)p * cap + i] = k[p];
+      if (k[p] != k0[p]) keys[i * (2 * W) + (u64)p] = k[p];
     if (__double_as_longlong(c) != __double_as_longlong(c0)) coef[i] = c;
   }
   const u64 bk = block_sum_u64(kills, lds_sum);
@@ -1183,8 +1213,8 @@ __global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ planes,
 
 // ---- k_overlap: sum of coefficients of terms with an all-zero x plane ---------
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ planes,
-                                                    const double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ keys,
+                                                    const double* __restrict__ coef,
                                                     const DevState* __restrict__ st, u32 in_slot,
                                                     double* __restrict__ partials) {
   __shared__ double lds_d[kBlock];
@@ -1194,7 +1224,7 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ plan
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
     u64 xo = 0;
 #pragma unroll
-    for (int p = 0; p < W; ++p) xo |= planes[(u64)p * cap + i];
+    for (int p = 0; p < W; ++p) xo |= keys[i * (2 * W) + (u64)p];
     if (xo == 0) {
       const double c = coef[i];
       if (!is_dead(c)) acc = __dadd_rn(acc, c);
@@ -1210,8 +1240,8 @@ __global__ __launch_bounds__(kBlock) void k_overlap(const u64* __restrict__ plan
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_trace(const u64* __restrict__ planes,
-                                                  const double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_trace(const u64* __restrict__ keys,
+                                                  const double* __restrict__ coef,
                                                   const DevState* __restrict__ st, u32 in_slot,
                                                   PatternProg prog, double* __restrict__ partials) {
   __shared__ double lds_d[kBlock];
@@ -1222,7 +1252,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(const u64* __restrict__ planes
     const double c = coef[i];
     if (is_dead(c)) continue;
     u64 k[2 * W];
-    load_key<W>(planes, cap, i, k);
+    akey_load<W>(keys, i, k);
     if (pattern_match<W>(k, prog)) acc = __dadd_rn(acc, c);
   }
   lds_d[threadIdx.x] = acc;
@@ -1239,8 +1269,8 @@ __global__ __launch_bounds__(kBlock) void k_trace(const u64* __restrict__ planes
 // a's pair table under a fresh generation.  Live keys of an arena are unique,
 // and the table holds at least twice the arena capacity, so each probe ends.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_ovl_build(const u64* __restrict__ planes,
-                                                      const double* __restrict__ coef, u64 cap,
+__global__ __launch_bounds__(kBlock) void k_ovl_build(const u64* __restrict__ keys,
+                                                      const double* __restrict__ coef,
                                                       const DevState* __restrict__ st, u32 in_slot,
                                                       u64* table, u64 tmask, u32 tgen) {
   const u64 n = st->slot[in_slot].n_entries;
@@ -1248,7 +1278,7 @@ __global__ __launch_bounds__(kBlock) void k_ovl_build(const u64* __restrict__ pl
   for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
     if (is_dead(coef[i])) continue;
     u64 k[2 * W];
-    load_key<W>(planes, cap, i, k);
+    akey_load<W>(keys, i, k);
     const u64 h = hash_key<W>(k);
     const u64 mine = ((u64)tgen << 40) | ((h >> 56) << 32) | i;
     u64 pos = h & tmask;
@@ -1268,11 +1298,11 @@ __global__ __launch_bounds__(kBlock) void k_ovl_build(const u64* __restrict__ pl
 
 // Probe pass over b: per-block partial sums of c_b * c_a for keys present in a.
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_ovl_probe(const u64* __restrict__ pa,
-                                                      const double* __restrict__ ca, u64 cap_a,
+__global__ __launch_bounds__(kBlock) void k_ovl_probe(const u64* __restrict__ ka,
+                                                      const double* __restrict__ ca,
                                                       const u64* __restrict__ table, u64 tmask, u32 tgen,
-                                                      const u64* __restrict__ pb,
-                                                      const double* __restrict__ cb, u64 cap_b,
+                                                      const u64* __restrict__ kb,
+                                                      const double* __restrict__ cb,
                                                       const DevState* __restrict__ st_b, u32 slot_b,
                                                       double* __restrict__ partials) {
   __shared__ double lds_d[kBlock];
@@ -1283,7 +1313,7 @@ __global__ __launch_bounds__(kBlock) void k_ovl_probe(const u64* __restrict__ pa
     const double c = cb[i];
     if (is_dead(c)) continue;
     u64 k[2 * W];
-    load_key<W>(pb, cap_b, i, k);
+    akey_load<W>(kb, i, k);
     const u64 h = hash_key<W>(k);
     const u64 fp = h >> 56;
     u64 pos = h & tmask;
@@ -1293,7 +1323,7 @@ __global__ __launch_bounds__(kBlock) void k_ovl_probe(const u64* __restrict__ pa
       if (((s >> 32) & 0xffull) == fp) {
         const u64 j = s & 0xffffffffull;
         u64 kj[2 * W];
-        load_key<W>(pa, cap_a, j, kj);
+        akey_load<W>(ka, j, kj);
         bool same = true;
 #pragma unroll
         for (int p = 0; p < 2 * W; ++p) same &= (kj[p] == k[p]);
@@ -1339,10 +1369,10 @@ __global__ __launch_bounds__(kBlock) void k_compact_count(const double* __restri
 }
 
 template <int W>
-__global__ __launch_bounds__(kBlock) void k_compact_write(const u64* __restrict__ sp,
+__global__ __launch_bounds__(kBlock) void k_compact_write(const u64* __restrict__ sk,
                                                           const double* __restrict__ sc,
-                                                          u64* __restrict__ dp, double* __restrict__ dc,
-                                                          u64 cap, DevState* st, u32 in_slot, u32 out_slot,
+                                                          u64* __restrict__ dk, double* __restrict__ dc,
+                                                          DevState* st, u32 in_slot, u32 out_slot,
                                                           const u64* __restrict__ masks,
                                                           const u64* __restrict__ cnt) {
   __shared__ u64 lds[kWaves];
@@ -1370,8 +1400,9 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const u64* __restrict_
     if ((m >> lane) & 1ull) {
       const u64 i = lo + (u64)j * 64 + lane;
       const u64 dst = base_off + loff[j] + (u64)__popcll(m & lanes_below(lane));
-#pragma unroll
-      for (int p = 0; p < 2 * W; ++p) dp[(u64)p * cap + dst] = sp[(u64)p * cap + i];
+      u64 k[2 * W];
+      akey_load<W>(sk, i, k);
+      akey_store<W>(dk, dst, k);
       dc[dst] = sc[i];
     }
   }
@@ -1383,16 +1414,14 @@ __global__ void k_fill_u64(u64* p, u64 n, u64 v) {
   for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
 }
 
-// Grow: copy the first n entries of every plane and the coefficients from an
-// arena of capacity cap_old into one of capacity cap_new.
-__global__ void k_copy_arena(const u64* __restrict__ sp, const double* __restrict__ sc, u64 cap_old,
-                             u64* __restrict__ dp, double* __restrict__ dc, u64 cap_new, u32 nplanes,
-                             u64 n) {
+// Grow / clone: copy the first n entries (nw key words each, then the
+// coefficients) into another arena; key arenas do not depend on the capacity.
+__global__ void k_copy_arena(const u64* __restrict__ sk, const double* __restrict__ sc,
+                             u64* __restrict__ dk, double* __restrict__ dc, u32 nw, u64 n) {
   const u64 stride = (u64)gridDim.x * blockDim.x;
-  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    for (u32 p = 0; p < nplanes; ++p) dp[(u64)p * cap_new + i] = sp[(u64)p * cap_old + i];
-    dc[i] = sc[i];
-  }
+  const u64 nk = n * nw;
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < nk; i += stride) dk[i] = sk[i];
+  for (u64 i = (u64)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dc[i] = sc[i];
 }
 
 }  // namespace ppvm
