@@ -730,10 +730,90 @@ def run_example(args) -> dict:
     }
 
 
+# ---- HBM traffic counters (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE) ----------------------
+CAL_ELEMENTS = 1 << 28
+
+
+def run_calibrate(args) -> dict:
+    """Calibration workload for the traffic counters: one elementwise kernel of
+    known traffic (b = a * 1 over 2^28 float32: 1 GiB read, 1 GiB written),
+    three times.  Its FETCH_SIZE / WRITE_SIZE readings per dispatch, divided
+    into 1 GiB, are the factors that turn the same counters into bytes."""
+    import torch
+    a = torch.ones(CAL_ELEMENTS, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 1.0, out=b)
+    torch.cuda.synchronize()
+    assert float(b[-1]) == 1.0
+    return {"metric": "counter calibration", "kernel": "elementwise mul (b = a * 1)", "elements": CAL_ELEMENTS,
+            "bytes_read": 4 * CAL_ELEMENTS, "bytes_written": 4 * CAL_ELEMENTS, "dispatches": 3}
+
+
+def _short_kernel(name: str) -> str:
+    import re
+    m = re.search(r"\b(k_[A-Za-z0-9_]+)", name)
+    return m.group(1) if m else name.split("(")[0][-80:]
+
+
+def counter_rows(root: str, counter: str, last_gates: int = 0) -> dict:
+    """kernel -> (dispatches, mean counter value) from the counter_collection
+    CSVs rocprofv3 wrote under root; with last_gates > 0 only the dispatches of
+    the last `last_gates` rotation gates (from the k_rot_count before the
+    last_gates-th last k_rot_gather on)."""
+    import csv
+    import glob
+    from collections import defaultdict
+    rows = []
+    for f in sorted(glob.glob(os.path.join(root, "**", "*counter_collection.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") == counter:
+                    rows.append((int(r["Dispatch_Id"]), _short_kernel(r["Kernel_Name"]), float(r["Counter_Value"])))
+    rows.sort()
+    if last_gates:
+        gi = [i for i, r in enumerate(rows) if r[1] == "k_rot_gather"]
+        start = gi[-last_gates] if len(gi) >= last_gates else 0
+        while start > 0 and rows[start][1] != "k_rot_count":
+            start -= 1
+        rows = rows[start:]
+    acc = defaultdict(lambda: [0, 0.0])
+    for _, k, v in rows:
+        acc[k][0] += 1
+        acc[k][1] += v
+    return {k: (n, tot / n) for k, (n, tot) in acc.items()}
+
+
+def run_traffic(args) -> dict:
+    """HBM bytes per dispatch from four rocprofv3 --pmc runs (calibration and
+    workload, FETCH_SIZE and WRITE_SIZE): each counter scaled by 1 GiB over
+    what it read for the calibration kernel."""
+    cf = counter_rows(args.cal_fetch, "FETCH_SIZE")
+    cw = counter_rows(args.cal_write, "WRITE_SIZE")
+    cal = next(k for k in cf if "MulFunctor" in k or "mul" in k.lower())
+    kf = (4 * CAL_ELEMENTS / 1024.0) / cf[cal][1]  # KiB per counter unit
+    kw = (4 * CAL_ELEMENTS / 1024.0) / cw[cal][1]
+    f = counter_rows(args.fetch, "FETCH_SIZE", args.last_gates)
+    w = counter_rows(args.write, "WRITE_SIZE", args.last_gates)
+    out = {}
+    for k in sorted(f, key=lambda k: -f[k][1] * f[k][0]):
+        if not k.startswith("k_") or k not in w:
+            continue
+        out[k] = {"dispatches": f[k][0], "fetch_KiB": f[k][1], "write_KiB": w[k][1],
+                  "hbm_bytes": (f[k][1] * kf + w[k][1] * kw) * 1024}
+    return {"metric": "HBM bytes per dispatch (rocprofv3 FETCH_SIZE, WRITE_SIZE, calibrated)",
+            "fetch_factor": kf, "write_factor": kw, "last_gates": args.last_gates, "kernels": out}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity", "lossy", "example", "configs"],
-                    default="lattice")
+    ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity", "lossy", "example", "configs",
+                                       "calibrate", "traffic"], default="lattice")
+    ap.add_argument("--cal-fetch", help="traffic: rocprofv3 output dir of the calibration FETCH_SIZE run")
+    ap.add_argument("--cal-write", help="traffic: ... calibration WRITE_SIZE run")
+    ap.add_argument("--fetch", help="traffic: ... workload FETCH_SIZE run")
+    ap.add_argument("--write", help="traffic: ... workload WRITE_SIZE run")
+    ap.add_argument("--last-gates", type=int, default=0, help="traffic: only the last N rotation gates")
     ap.add_argument("--verify", type=int, default=16, help="batched: instances re-run on the engine")
     ap.add_argument("--cpu-instances", type=int, default=256, help="batched: at most this many instances on the CPU")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="batched: CPU baseline time budget (s)")
@@ -749,7 +829,7 @@ def main() -> int:
         return 2
     res = {"lattice": run, "batched": run_batched, "surrogate": run_surrogate,
            "parity": run_parity, "lossy": run_lossy, "example": run_example,
-           "configs": run_configs}[args.mode](args)
+           "configs": run_configs, "calibrate": run_calibrate, "traffic": run_traffic}[args.mode](args)
     print(json.dumps(res), flush=True)
     return 0
 

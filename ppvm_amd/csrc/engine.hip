@@ -404,6 +404,9 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
                        s->cap, lgm, it.seq + 1, hist);
     hipLaunchKernelGGL(k_pair_colscan, dim3((1u << lgm) / 64 + 1), dim3(kPartThreads), 0, s->stream, s->st, in,
                        s->cap, lgm, hist, bmeta);
+    // one of the two scatters does the work (by the gate's bucket count)
+    hipLaunchKernelGGL(k_pair_scatter_tiled, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st,
+                       in, s->cap, lgm, hist, s->bkt, bmeta);
     hipLaunchKernelGGL(k_pair_scatter, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st, in,
                        s->cap, lgm, hist, s->bkt, bmeta);
     hipLaunchKernelGGL(k_pair_lds<W>, dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, keys, wl, s->st, in,

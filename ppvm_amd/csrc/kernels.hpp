@@ -626,6 +626,120 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_colscan(const DevState* _
   }
 }
 
+// ---- k_pair_scatter_tiled ---------------------------------------------------------
+// The scatter for up to 2^kTiledLgMax buckets: slice s takes its records in
+// tiles of kScatterTile, sorts each tile by bucket in LDS (local counts, scan,
+// placement) and then writes the tile in that order, so consecutive threads
+// store consecutive addresses of one bucket's run instead of one 8-byte store
+// per bucket line (k_pair_scatter, used for more buckets, does the latter).
+// Results are the same multiset per bucket; the order inside a bucket does
+// not matter to k_pair_lds.
+constexpr u32 kTiledLgMax = 12;
+constexpr u32 kScatterTile = 8192;
+constexpr u32 kTilePer = kScatterTile / kPartThreads;
+constexpr u32 kTiledBuckets = 1u << kTiledLgMax;
+constexpr u32 kTiledPer = kTiledBuckets / kPartThreads;
+__global__ __launch_bounds__(kPartThreads) void k_pair_scatter_tiled(const u64* __restrict__ hsh,
+                                                                    const DevState* __restrict__ st, u32 in_slot,
+                                                                    u64 cap, u32 lg_max,
+                                                                    const u32* __restrict__ hist,
+                                                                    u64* __restrict__ bkt, u32* __restrict__ bmeta) {
+  __shared__ u32 off[kTiledBuckets];     // next global position per bucket (this slice)
+  __shared__ u32 tcnt[kTiledBuckets];    // records per bucket in the tile
+  __shared__ u32 tstart[kTiledBuckets];  // tile position of each bucket's first record
+  __shared__ u64 buf[kScatterTile];      // the tile in bucket order
+  __shared__ unsigned short bid[kScatterTile];
+  __shared__ u32 wsum[kPartThreads / 64];
+  __shared__ u32 ttot;
+  if (st->overflow) return;
+  const u64 n_entries = st->slot[in_slot].n_entries;
+  const u64 n_wl = st->n_wl;
+  if (n_entries + n_wl > cap) return;
+  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  if (lg > kTiledLgMax) return;  // k_pair_scatter places this gate's records
+  const u32 s = blockIdx.x, tid = threadIdx.x;
+  // bucket bases (exclusive scan of the sizes; block 0 publishes them) and this
+  // slice's offset inside each bucket
+  {
+    const u32 b0 = tid * kTiledPer;
+    const u32* row = hist + (u64)s * nb;
+    u32 loc[kTiledPer], pre[kTiledPer], sum = 0;
+#pragma unroll
+    for (u32 q = 0; q < kTiledPer; ++q) {
+      const bool on = b0 + q < nb;
+      loc[q] = on ? bmeta[kMaxBuckets + b0 + q] : 0u;
+      pre[q] = on ? row[b0 + q] : 0u;
+      sum += loc[q];
+    }
+    u32 base = block_excl_scan_u32(sum, wsum);
+#pragma unroll
+    for (u32 q = 0; q < kTiledPer; ++q) {
+      if (b0 + q < nb) {
+        if (s == 0) bmeta[b0 + q] = base;
+        off[b0 + q] = base + pre[q];
+      }
+      base += loc[q];
+    }
+  }
+  const u64 lo = n_wl * s / kPairSlices, hi = n_wl * (s + 1) / kPairSlices;
+  for (u64 t0 = lo; t0 < hi; t0 += kScatterTile) {
+    const u32 tn = (u32)min((u64)kScatterTile, hi - t0);
+    for (u32 b = tid; b < nb; b += kPartThreads) tcnt[b] = 0u;
+    __syncthreads();
+    u64 myv[kTilePer];
+    u32 myb[kTilePer], myr[kTilePer];
+#pragma unroll
+    for (u32 q = 0; q < kTilePer; ++q) {
+      const u32 e = tid + q * kPartThreads;
+      myb[q] = 0xffffffffu;
+      if (e < tn) {
+        const u64 v = hsh[t0 + e];
+        if (v != kDeadHash) {
+          const u32 b = bucket_of(v, lg);
+          myb[q] = b;
+          myr[q] = atomicAdd(&tcnt[b], 1u);
+          myv[q] = (t0 + e) | ((u64)(u32)v << 32);
+        }
+      }
+    }
+    __syncthreads();
+    {  // tile positions: exclusive scan of the tile counts
+      const u32 b0 = tid * kTiledPer;
+      u32 c[kTiledPer], sum = 0;
+#pragma unroll
+      for (u32 q = 0; q < kTiledPer; ++q) {
+        c[q] = b0 + q < nb ? tcnt[b0 + q] : 0u;
+        sum += c[q];
+      }
+      u32 base = block_excl_scan_u32(sum, wsum);
+#pragma unroll
+      for (u32 q = 0; q < kTiledPer; ++q) {
+        if (b0 + q < nb) tstart[b0 + q] = base;
+        base += c[q];
+      }
+      if (tid == kPartThreads - 1u) ttot = base;
+    }
+    __syncthreads();
+#pragma unroll
+    for (u32 q = 0; q < kTilePer; ++q) {
+      if (myb[q] != 0xffffffffu) {
+        const u32 p = tstart[myb[q]] + myr[q];
+        buf[p] = myv[q];
+        bid[p] = (unsigned short)myb[q];
+      }
+    }
+    __syncthreads();
+    const u32 tot = ttot;
+    for (u32 p = tid; p < tot; p += kPartThreads) {
+      const u32 b = bid[p];
+      bkt[(u64)off[b] + (p - tstart[b])] = buf[p];
+    }
+    __syncthreads();
+    for (u32 b = tid; b < nb; b += kPartThreads) off[b] += tcnt[b];
+    __syncthreads();
+  }
+}
+
 // ---- k_pair_scatter -------------------------------------------------------------
 // Slice s writes bkt[pos] = record | (low 32 bits of h) << 32, at its offset
 // in the record's bucket: bucket base (prefix of the bucket sizes) plus the
@@ -642,6 +756,7 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_scatter(const u64* __rest
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
   const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  if (lg <= kTiledLgMax) return;  // k_pair_scatter_tiled places this gate's records
   const u32 s = blockIdx.x;
   // bucket bases: exclusive scan of the sizes, kPer consecutive buckets per thread
   constexpr u32 kPer = kMaxBuckets / kPartThreads;
