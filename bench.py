@@ -343,7 +343,8 @@ def run_batched(args) -> dict:
     runs = []
     for _ in range(max(1, args.steps)):
         r = run_hybrid(b, threads=best["threads"], use_graph=best["graph"],
-                       engine_threads=best["engine_threads"], engine_set=best["engine_set"])
+                       engine_threads=best["engine_threads"], engine_set=best["engine_set"],
+                       segment_ops=best["segment_ops"], blocks=best["blocks"])
         runs.append(r)
         _log(f"batched run: {r.wall_s:.3f}s kernel {r.batch_ms:.0f} ms, {len(r.on_engine)} on engine")
     r = min(runs, key=lambda x: x.wall_s)
@@ -357,6 +358,19 @@ def run_batched(args) -> dict:
         ln, ov, _, _ = _engine_run(ws[i])
         len_ok &= bool(np.array_equal(np.asarray(r.lens[i], dtype=np.uint64), ln))
         max_dov = max(max_dov, float(np.max(np.abs(np.asarray(r.overlaps[i]) - ov))))
+    # every instance against the oracle's committed per-layer marks
+    fix_path = os.path.join(ROOT, "tests", "golden", "batched_chains_marks.json")
+    fixture = None
+    if os.path.exists(fix_path):
+        with open(fix_path) as f:
+            fixture = json.load(f)["instances"]
+    if fixture is not None:
+        lens_all = all(np.array_equal(np.asarray(r.lens[i], dtype=np.uint64),
+                                      np.asarray(fixture[i]["lens"], dtype=np.uint64)) for i in range(len(ws)))
+        ovl_all = max(float(np.max(np.abs(np.asarray(r.overlaps[i]) - np.asarray(fixture[i]["overlaps"]))))
+                      for i in range(len(ws)))
+        tu_all = all(int(r.term_updates[i]) == fixture[i]["term_updates"] for i in range(len(ws)))
+    in_kernel = [i for i in range(len(ws)) if i not in set(r.on_engine)]
     out = {
         "metric": "batched term-updates/s, 256 TFIM chains",
         "value": U / r.wall_s,
@@ -374,12 +388,18 @@ def run_batched(args) -> dict:
                    "autotune_s": tune_s, "autotune_trials": best.get("trials")},
         "latency_s": {"p50": float(np.percentile(r.latency_s, 50)),
                       "p99": float(np.percentile(r.latency_s, 99)),
-                      "max": float(np.max(r.latency_s))},
+                      "max": float(np.max(r.latency_s)),
+                      "persistent_kernel_instances": len(in_kernel),
+                      "persistent_kernel_p50": float(np.percentile(r.latency_s[in_kernel], 50)) if in_kernel else None,
+                      "persistent_kernel_p99": float(np.percentile(r.latency_s[in_kernel], 99)) if in_kernel else None},
         "kernel_ms": r.batch_ms,
         "on_engine": len(r.on_engine),
         "parity_vs_sequential_engine": {"instances": len(sample), "layer_counts_equal": len_ok,
                                         "max_overlap_abs_diff": max_dov},
     }
+    if fixture is not None:
+        out["parity_vs_oracle_marks"] = {"instances": len(ws), "layer_counts_equal": bool(lens_all),
+                                         "term_updates_equal": bool(tu_all), "max_overlap_abs_diff": ovl_all}
     if not args.no_cpu:
         from oracle.pyoracle import OracleSum
         # random-order sample of batch instances within a CPU time budget; an

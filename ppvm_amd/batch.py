@@ -227,9 +227,11 @@ class HybridResult:
     engine_s: float
     wall_s: float
     latency_s: np.ndarray     # per-instance completion time from the start of the run
+    digests: list             # per engine instance: ppvm_fingerprint of its final terms (None in the kernel)
 
 
-def _engine_run(w):
+def _engine_run_digest(w):
+    """_engine_run plus the ppvm_fingerprint digest of the final terms."""
     from .paulisum import PauliSum
     g = PauliSum.new(w.n_qubits, w.observable, min_abs_coeff=w.min_abs_coeff,
                      max_pauli_weight=w.max_pauli_weight)
@@ -243,13 +245,19 @@ def _engine_run(w):
     if prev < len(w.program):
         g.apply(w.program[prev:])
     g.synchronize()
-    tu = g.stats()["term_updates"]
-    pk = g.stats()["peak_live"]
-    return np.array(lens, dtype=np.uint64), np.array(ovls), tu, pk
+    st = g.stats()
+    return np.array(lens, dtype=np.uint64), np.array(ovls), st["term_updates"], st["peak_live"], g._h.fingerprint()
+
+
+def _engine_run(w):
+    """One workload on the full-GPU engine: term counts and overlaps at its
+    layer marks, term updates, peak live terms."""
+    lens, ovls, tu, pk, _ = _engine_run_digest(w)
+    return lens, ovls, tu, pk
 
 
 def run_hybrid(batch: Batch, threads: int = 256, use_graph: bool = True, engine_threads: int = 4,
-               order=None, engine_set=None) -> HybridResult:
+               order=None, engine_set=None, segment_ops: int = 0, blocks: int = 0) -> HybridResult:
     """Run every instance.  Instances in ``engine_set`` start on the full-GPU
     engine at once (``engine_threads`` host threads, one stream each) while
     the persistent kernel runs the rest; instances that overflow the batch
@@ -264,6 +272,7 @@ def run_hybrid(batch: Batch, threads: int = 256, use_graph: bool = True, engine_
     tu = np.zeros(n, dtype=np.int64)
     pk = np.zeros(n, dtype=np.int64)
     lat = np.zeros(n)
+    digests = [None] * n
     lock = threading.Lock()
     todo = list(engine_set)
     errors = []
@@ -276,18 +285,19 @@ def run_hybrid(batch: Batch, threads: int = 256, use_graph: bool = True, engine_
                     return
                 i = todo.pop(0)
             try:
-                ln, ov, t, p = _engine_run(batch.workloads[i])
+                ln, ov, t, p, dg = _engine_run_digest(batch.workloads[i])
             except Exception as e:  # surfaced after join
                 errors.append(e)
                 return
             with lock:
-                lens[i], overlaps[i], tu[i], pk[i] = ln, ov, t, p
+                lens[i], overlaps[i], tu[i], pk[i], digests[i] = ln, ov, t, p, dg
                 lat[i] = time.perf_counter() - t0
 
     ths = [threading.Thread(target=worker) for _ in range(max(1, min(engine_threads, len(todo))))]
     for t in ths:
         t.start()
-    ms = batch.run(threads=threads, use_graph=use_graph, order=order) if order else 0.0
+    ms = batch.run(threads=threads, blocks=blocks, segment_ops=segment_ops, use_graph=use_graph,
+                   order=order) if order else 0.0
     res = batch.results()
     dev_lat = batch.latency()
     redo = [i for i in order if res["status"][i] != 0]
@@ -310,19 +320,24 @@ def run_hybrid(batch: Batch, threads: int = 256, use_graph: bool = True, engine_
     if errors:
         raise errors[0]
     t_end = time.perf_counter()
-    return HybridResult(overlaps, lens, tu, pk, list(engine_set) + redo, ms, t_end - te, t_end - t0, lat)
+    return HybridResult(overlaps, lens, tu, pk, list(engine_set) + redo, ms, t_end - te, t_end - t0, lat, digests)
 
 
-def autotune(workloads, caps=(1 << 16, 1 << 18), threads=(256, 1024), graphs=(True,),
-             engine_threads=(4, 8, 16), cache_path: str | None = None, log=None) -> dict:
-    """Measure candidate settings on the batch itself and return the fastest.
+def autotune(workloads, caps=(1 << 16, 1 << 18, 1 << 20), threads=(256, 512, 1024), segments=(0, 256, 2048),
+             blocks=(0, 1024), engine_threads=(8, 16), graphs=(True,), cache_path: str | None = None,
+             log=None) -> dict:
+    """Choose the batched run's settings by measuring them on the batch itself.
 
-    For each arena cap the first trial runs everything through the persistent
-    kernel and learns which instances outgrow the cap; later trials start
-    those on the engine concurrently with the kernel.  Candidates: arena cap,
-    workgroup size, hipGraph capture, engine host threads.  The winner (with
-    its engine set) is cached in ``cache_path`` (JSON) keyed by a signature of
-    the workloads."""
+    Searched: the per-instance arena cap (instances that outgrow it run on the
+    full-GPU engine, concurrently with the persistent kernel), the workgroup
+    size (the tile of terms one pass of a workgroup covers), the segment size
+    (ops per kernel launch of the captured launch sequence; 0 = whole
+    programs), the persistent grid (0 = two workgroups per CU) and the engine's
+    host threads.  For each cap a first run through the kernel alone learns
+    which instances outgrow it; then one setting is varied at a time from the
+    current best (coordinate search), so the trials grow with the sum, not the
+    product, of the candidate counts.  The winner (with its engine set) is
+    cached in ``cache_path`` (JSON) keyed by a signature of the workloads."""
     sig = f"{len(workloads)}:{sum(w.n_gates for w in workloads)}:{sum(w.n_qubits for w in workloads)}"
     if cache_path and os.path.exists(cache_path):
         try:
@@ -331,25 +346,39 @@ def autotune(workloads, caps=(1 << 16, 1 << 18), threads=(256, 1024), graphs=(Tr
                 return cached[sig]
         except (OSError, ValueError):
             pass
-    best = None
     trials = []
+    best = None
+
+    def trial(b, big, cfg):
+        nonlocal best
+        r = run_hybrid(b, threads=cfg["threads"], use_graph=cfg["graph"], engine_threads=cfg["engine_threads"],
+                       engine_set=big, segment_ops=cfg["segment_ops"], blocks=cfg["blocks"])
+        t = dict(cfg, wall_s=r.wall_s, batch_ms=r.batch_ms, on_engine=len(r.on_engine), engine_set=big)
+        trials.append(t)
+        if log:
+            log("autotune " + " ".join(f"{k}={cfg[k]}" for k in cfg) +
+                f": {r.wall_s:.3f}s (kernel {r.batch_ms:.0f} ms, {len(r.on_engine)} on engine)")
+        if best is None or r.wall_s < best["wall_s"]:
+            best = dict(t)
+        return r.wall_s
+
+    base = {"threads": threads[-1], "graph": graphs[0], "engine_threads": engine_threads[-1],
+            "segment_ops": segments[0], "blocks": blocks[0]}
     for cap in caps:
         b = Batch(workloads, capacity=cap)
-        first = run_hybrid(b, threads=max(threads), use_graph=False)
+        first = run_hybrid(b, threads=base["threads"], use_graph=False)
         big = sorted(first.on_engine)
-        for th in threads:
-            for gr in graphs:
-                for et in engine_threads:
-                    r = run_hybrid(b, threads=th, use_graph=gr, engine_threads=et, engine_set=big)
-                    t = {"cap": cap, "threads": th, "graph": gr, "engine_threads": et,
-                         "wall_s": r.wall_s, "batch_ms": r.batch_ms, "on_engine": len(r.on_engine),
-                         "engine_set": big}
-                    trials.append(t)
-                    if log:
-                        log(f"autotune cap={cap} threads={th} graph={gr} engine_threads={et}: "
-                            f"{r.wall_s:.3f}s (kernel {r.batch_ms:.0f} ms, {len(r.on_engine)} on engine)")
-                    if best is None or r.wall_s < best["wall_s"]:
-                        best = dict(t)
+        cur = dict(base, cap=cap)
+        cur_s = trial(b, big, cur)
+        for key, cands in (("threads", threads), ("segment_ops", segments), ("blocks", blocks),
+                           ("engine_threads", engine_threads), ("graph", graphs)):
+            for v in cands:
+                if v == cur[key]:
+                    continue
+                cfg = dict(cur, **{key: v})
+                s_ = trial(b, big, cfg)
+                if s_ < cur_s:
+                    cur, cur_s = cfg, s_
         b.close()
     out = dict(best, trials=[{k: v for k, v in t.items() if k != "engine_set"} for t in trials],
                signature=sig)

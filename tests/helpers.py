@@ -137,3 +137,18 @@ def layer_fingerprints(sum_, w, export_fn) -> list:
         fp["overlap"] = float(sum_.overlap_with_zero())
         out.append(fp)
     return out
+
+
+def host_fingerprint(x, z, c):
+    """ppvm_fingerprint's definition in numpy: per term, d = m(...m(m(s ^ x_0) ^ x_1)... ^ z_{W-1})
+    with m the SplitMix64 finaliser, e = m(d ^ coefficient bits); (count, sum d, sum e) mod 2^64."""
+    def m(v):
+        v = (v ^ (v >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        v = (v ^ (v >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return v ^ (v >> np.uint64(31))
+    with np.errstate(over="ignore"):
+        d = np.full(len(c), np.uint64(0x9E3779B97F4A7C15))
+        for col in list(np.asarray(x, dtype=np.uint64).T) + list(np.asarray(z, dtype=np.uint64).T):
+            d = m(d ^ col)
+        e = m(d ^ np.ascontiguousarray(c, dtype=np.float64).view(np.uint64))
+        return (len(c), int(d.sum(dtype=np.uint64)), int(e.sum(dtype=np.uint64)))

@@ -23,24 +23,9 @@ import pytest
 from oracle.pyoracle import OracleSum
 from ppvm_amd import ops as O
 from ppvm_amd import workloads as WL
-from tests.helpers import gpu_for, oracle_for
+from tests.helpers import gpu_for, host_fingerprint, oracle_for
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lattice_full_fingerprints.json")
-
-
-def host_fingerprint(x, z, c):
-    """ppvm_fingerprint's definition in numpy: per term, d = m(...m(m(s ^ x_0) ^ x_1)... ^ z_{W-1})
-    with m the SplitMix64 finaliser, e = m(d ^ coefficient bits); (count, sum d, sum e) mod 2^64."""
-    def m(v):
-        v = (v ^ (v >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        v = (v ^ (v >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        return v ^ (v >> np.uint64(31))
-    with np.errstate(over="ignore"):
-        d = np.full(len(c), np.uint64(0x9E3779B97F4A7C15))
-        for col in list(np.asarray(x, dtype=np.uint64).T) + list(np.asarray(z, dtype=np.uint64).T):
-            d = m(d ^ col)
-        e = m(d ^ np.ascontiguousarray(c, dtype=np.float64).view(np.uint64))
-        return (len(c), int(d.sum(dtype=np.uint64)), int(e.sum(dtype=np.uint64)))
 
 
 def _golden():
