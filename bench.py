@@ -825,10 +825,48 @@ def run_traffic(args) -> dict:
             "fetch_factor": kf, "write_factor": kw, "last_gates": args.last_gates, "kernels": out}
 
 
+def run_kstats(args) -> dict:
+    """Per-kernel statistics from a rocprofv3 --kernel-trace --stats run
+    (the *kernel_stats.csv under --dir), durations in microseconds."""
+    import csv
+    import glob
+    rows = []
+    for f in sorted(glob.glob(os.path.join(args.dir, "**", "*kernel_stats.csv"), recursive=True)):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                name = r["Name"]
+                k = _short_kernel(name)
+                if "<" in name.split("(")[0] and k.startswith("k_"):
+                    k += "<" + name.split("(")[0].split("<", 1)[1]
+                rows.append({"kernel": k, "calls": int(r["Calls"]), "total_ms": float(r["TotalDurationNs"]) / 1e6,
+                             "avg_us": float(r["AverageNs"]) / 1e3, "pct": float(r["Percentage"]),
+                             "min_us": float(r["MinNs"]) / 1e3, "max_us": float(r["MaxNs"]) / 1e3})
+    rows.sort(key=lambda r: -r["total_ms"])
+    return {"metric": "rocprofv3 kernel statistics", "kernels": rows}
+
+
+def print_table(res: dict) -> None:
+    """Text table of a kstats or traffic result (what profiles/ holds)."""
+    if res["metric"].startswith("rocprofv3 kernel"):
+        print(f"{'kernel':<26} {'calls':>7} {'total_ms':>10} {'avg_us':>9} {'pct':>6} {'min_us':>8} {'max_us':>9}")
+        for r in res["kernels"]:
+            print(f"{r['kernel']:<26} {r['calls']:>7d} {r['total_ms']:>10.2f} {r['avg_us']:>9.2f} {r['pct']:>6.2f} "
+                  f"{r['min_us']:>8.2f} {r['max_us']:>9.2f}")
+        return
+    print(f"# HBM traffic per dispatch: FETCH_SIZE x {res['fetch_factor']:.4f} + WRITE_SIZE x "
+          f"{res['write_factor']:.4f} (KiB -> bytes x 1024)" +
+          (f"; last {res['last_gates']} rotation gates only" if res["last_gates"] else ""))
+    print(f"{'kernel':<24} {'dispatches':>10} {'fetch_KiB':>12} {'write_KiB':>12} {'hbm_bytes':>14}")
+    for k, v in res["kernels"].items():
+        print(f"{k:<24} {v['dispatches']:>10d} {v['fetch_KiB']:>12.1f} {v['write_KiB']:>12.1f} {v['hbm_bytes']:>14.0f}")
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity", "lossy", "example", "configs",
-                                       "calibrate", "traffic"], default="lattice")
+                                       "calibrate", "traffic", "kstats"], default="lattice")
+    ap.add_argument("--dir", help="kstats: rocprofv3 output dir of a --kernel-trace --stats run")
+    ap.add_argument("--table", action="store_true", help="traffic / kstats: print a text table, not JSON")
     ap.add_argument("--cal-fetch", help="traffic: rocprofv3 output dir of the calibration FETCH_SIZE run")
     ap.add_argument("--cal-write", help="traffic: ... calibration WRITE_SIZE run")
     ap.add_argument("--fetch", help="traffic: ... workload FETCH_SIZE run")
@@ -849,7 +887,11 @@ def main() -> int:
         return 2
     res = {"lattice": run, "batched": run_batched, "surrogate": run_surrogate,
            "parity": run_parity, "lossy": run_lossy, "example": run_example,
-           "configs": run_configs, "calibrate": run_calibrate, "traffic": run_traffic}[args.mode](args)
+           "configs": run_configs, "calibrate": run_calibrate, "traffic": run_traffic,
+           "kstats": run_kstats}[args.mode](args)
+    if args.table and args.mode in ("traffic", "kstats"):
+        print_table(res)
+        return 0
     print(json.dumps(res), flush=True)
     return 0
 
