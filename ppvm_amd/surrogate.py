@@ -5,8 +5,9 @@ An ensemble of MLPs (inputs: h, t, steps and sin/cos features; hidden layers
 accumulation on MFMA) is trained on the GPU, by the library's own forward and
 backward kernels, on expectation values produced by the GPU propagator.  At
 query time the ensemble mean is used where the ensemble agrees (standard
-deviation <= ``fallback_std``); elsewhere the propagator is run (one
-propagation per distinct h, prefix reuse over t).
+deviation within a bound calibrated on held-out validation points);
+elsewhere the propagator is run (one propagation per distinct h, prefix
+reuse over t).
 """
 
 from __future__ import annotations
@@ -159,87 +160,175 @@ def load_grid(path: str = GRID_FILE) -> dict:
 
 
 @dataclass
+class SweepSplit:
+    """Grid points of a sweep split for training, calibration and query.
+
+    Points are flat indices into the (h, t) grid.  ``pair`` names the physical
+    point (h index, step count): grid columns with equal ``round(t / dt)`` are
+    the same point.  Training points are the seed-sampled ones; a
+    validation split of their distinct pairs is held out of fitting and used
+    only to calibrate the fallback bound.  A query point is *known* when its
+    pair is a training pair (its value is then the training target) and *new*
+    otherwise; new pairs share nothing with the fitted or validation pairs."""
+    steps: np.ndarray      # steps per flat grid point
+    hidx: np.ndarray       # h index per flat grid point
+    pair: np.ndarray       # pair id per flat grid point
+    train: np.ndarray      # seed-sampled training points
+    fit: np.ndarray        # training points the ensemble is fitted on
+    val: np.ndarray        # training points held out for calibration (pairs disjoint from fit)
+    query: np.ndarray      # all other grid points
+    known: np.ndarray      # per query point: pair is a training pair
+
+
+def split_sweep(h_values, t_values, n_train: int = 2048, seed: int = 31, val_frac: float = 0.2,
+                dt: float = DT) -> SweepSplit:
+    nh, nt = len(h_values), len(t_values)
+    steps = np.broadcast_to(np.rint(np.asarray(t_values) / dt).astype(np.int64), (nh, nt)).ravel()
+    hidx = np.repeat(np.arange(nh), nt)
+    pair = hidx * (int(steps.max()) + 1) + steps
+    flat = np.arange(nh * nt)
+    train = np.sort(np.random.default_rng(seed).choice(flat, size=n_train, replace=False))
+    query = np.setdiff1d(flat, train)
+    tpairs = np.unique(pair[train])
+    vpairs = np.random.default_rng(seed + 1).choice(tpairs, size=int(round(val_frac * len(tpairs))),
+                                                    replace=False)
+    in_val = np.isin(pair[train], vpairs)
+    return SweepSplit(steps, hidx, pair, train, train[~in_val], train[in_val], query,
+                      np.isin(pair[query], tpairs))
+
+
+def calibrate_bound(sd: np.ndarray, err: np.ndarray, bar: float) -> float:
+    """Largest ensemble-std bound under which every validation point is within
+    ``bar`` of the propagator: the std of the last point, in increasing std,
+    before the first one outside the bar.  -1 when even the most confident
+    point misses (every query point then falls back)."""
+    order = np.argsort(sd, kind="stable")
+    bad = np.nonzero(err[order] > bar)[0]
+    k = len(order) if len(bad) == 0 else int(bad[0])
+    return float(sd[order[k - 1]]) if k > 0 else -1.0
+
+
+@dataclass
 class SweepReport:
     n_query: int
-    n_fallback: int
+    n_known: int               # query points whose (h, steps) is a training pair: answered by lookup
+    n_new: int
+    bound: float               # fallback bound on the ensemble std (calibrated unless given)
+    val_points: int
+    val_accepted: int          # validation points under the bound (all within the bar)
+    n_accepted: int            # new query points answered by the network
+    n_fallback: int            # new query points answered by a fallback propagation
     fallback_h: int
-    abs_err: np.ndarray        # per query point, vs the propagator
-    frac_within_1e3: float
+    abs_err: np.ndarray        # per query point, vs this run's propagator values
+    frac_within_bar: float
     max_err: float
     train_s: float
     infer_s: float
     fallback_s: float
-    surrogate_s: float         # inference + fallback propagations
-    plain_s: float             # propagator sweep over the same query points
+    surrogate_s: float         # whole query phase: lookup, inference, fallback propagations
+    plain_s: float             # live plain sweep of the same query points
     speedup: float
     train_loss: list
-    abs_err_net: np.ndarray    # network alone (before fallback), per query point
-    fallback_live: bool        # fallbacks propagated live (else charged at recorded times)
-    tradeoff: list             # per disagreement bound: fallbacks, accuracy, speedup
+    abs_err_net: np.ndarray    # network alone on the new query points
+    grid_max_diff: float       # |live plain sweep - training grid| over the points both hold
+    tradeoff: list             # other bounds, fallbacks charged at this run's plain prefix times
 
 
-def run_config5(grid: np.ndarray, h_values: np.ndarray, t_values: np.ndarray, plain_seconds: np.ndarray,
-                n_train: int = 2048, seed: int = 31, members: int = 4, steps: int = 40000, lr: float = 1e-3,
-                fallback_std: float = 1e-2, propagate=None, log=None) -> SweepReport:
-    """Surrogate sweep of the (h, t) grid: train on ``n_train`` seed-sampled
-    points, infer the rest, fall back to the propagator where the ensemble std
-    exceeds ``fallback_std``.  ``grid[i, m]`` is the propagator value for h_i
-    after m steps; ``plain_seconds[i]`` the propagator time for h_i over the
-    full step range (the plain sweep's cost); ``propagate(h, steps)`` runs the
-    propagator for fallbacks and returns the overlap series."""
-    Hh, Tt = np.meshgrid(h_values, t_values, indexing="ij")
-    steps_grid = np.rint(Tt / DT).astype(np.int64)
-    Y = grid[np.arange(len(h_values))[:, None], steps_grid]
-    flat = np.arange(Hh.size)
-    rng = np.random.default_rng(seed)
-    train_idx = np.sort(rng.choice(flat, size=n_train, replace=False))
-    query_idx = np.setdiff1d(flat, train_idx)
-    Xall = features(Hh.ravel(), Tt.ravel())
-    Yall = Y.ravel()
-    net = MLPEnsemble(Xall.shape[1], 256, members, max(len(train_idx), len(query_idx)), seed)
+def run_sweep(grid: np.ndarray, h_values, t_values, propagate, n_train: int = 2048, seed: int = 31,
+              members: int = 4, steps: int = 12000, lr: float = 2e-3, bar: float = 1e-3,
+              val_frac: float = 0.2, bound: float | None = None, log=None) -> SweepReport:
+    """Surrogate sweep of an (h, t) grid against the plain propagator sweep,
+    both timed live in this call.
+
+    ``grid[i, m]`` holds the propagator's value for ``h_values[i]`` after m
+    steps; only the training points are read from it.  ``propagate(h, m)``
+    runs the propagator and returns (values after 0..m steps, cumulative
+    seconds after 0..m steps).
+
+    1. Plain sweep (timed): one propagation per h value that holds a query
+       point, to its largest query step.  Its values judge every error below.
+    2. Fit the ensemble on the training points outside the validation split.
+    3. Bound (unless ``bound`` is given): the largest ensemble std under which
+       every validation point is within ``bar``.
+    4. Query (timed): known pairs by lookup; new pairs by the ensemble where
+       its std is within the bound; for every h holding a new pair above the
+       bound, one propagation to the largest such step, whose values answer
+       every query point of that h it covers."""
+    h_values = np.asarray(h_values, dtype=np.float64)
+    sp = split_sweep(h_values, t_values, n_train, seed, val_frac)
+    nq = len(sp.query)
+    qh, qs = sp.hidx[sp.query], sp.steps[sp.query]
+    # 1. plain sweep
+    q_h = np.unique(qh)
+    need = {int(i): int(qs[qh == i].max()) for i in q_h}
+    truth, cum = {}, {}
     t0 = time.perf_counter()
-    loss = net.train(Xall[train_idx], Yall[train_idx].astype(np.float32), steps=steps, lr=lr)
+    for n_done, i in enumerate(q_h):
+        truth[int(i)], cum[int(i)] = propagate(float(h_values[i]), need[int(i)])
+        if log and (n_done % 8 == 7 or n_done == len(q_h) - 1):
+            log(f"plain sweep {n_done + 1}/{len(q_h)} h values, {time.perf_counter() - t0:.1f}s")
+    plain_s = time.perf_counter() - t0
+    ref = np.array([truth[int(i)][m] for i, m in zip(qh, qs)])
+    grid_diff = max(float(np.max(np.abs(truth[i] - grid[i, :len(truth[i])]))) for i in truth)
+    # 2. fit
+    Xall = features(h_values[sp.hidx], sp.steps * DT)
+    Yall = grid[sp.hidx, sp.steps]
+    net = MLPEnsemble(Xall.shape[1], 256, members, max(len(sp.fit), nq), seed)
+    t0 = time.perf_counter()
+    loss = net.train(Xall[sp.fit], Yall[sp.fit].astype(np.float32), steps=steps, lr=lr)
     train_s = time.perf_counter() - t0
     if log:
-        log(f"surrogate trained: {train_s:.2f}s, loss per member {loss.tolist()}")
-    # timed surrogate sweep over the query points
+        log(f"ensemble fitted on {len(sp.fit)} points: {train_s:.2f}s, loss per member {loss.tolist()}")
+    # 3. bound
+    vm, vsd = net.predict(Xall[sp.val])
+    verr = np.abs(vm.astype(np.float64) - Yall[sp.val])
+    if bound is None:
+        bound = calibrate_bound(vsd, verr, bar)
+    val_acc = int(np.sum(vsd <= bound))
+    # 4. query
+    new = ~sp.known
+    lut = dict(zip(sp.pair[sp.train].tolist(), Yall[sp.train].tolist()))
+    pred = np.zeros(nq)
     t0 = time.perf_counter()
-    mean, sd = net.predict(Xall[query_idx])
+    pred[sp.known] = [lut[p] for p in sp.pair[sp.query[sp.known]].tolist()]
+    mean, sd = net.predict(Xall[sp.query[new]])
     infer_s = time.perf_counter() - t0
-    fb = sd > fallback_std
-    pred = mean.astype(np.float64)
-    err_net = np.abs(pred - Yall[query_idx])
-    hi = (query_idx // len(t_values))
-    fb_h = np.unique(hi[fb])
-    t0 = time.perf_counter()
+    pred[new] = mean
+    fb = np.zeros(nq, dtype=bool)
+    fb[np.nonzero(new)[0][sd > bound]] = True
+    fb_h = np.unique(qh[fb])
+    t1 = time.perf_counter()
     for n_done, i in enumerate(fb_h):
-        need = int(steps_grid.ravel()[query_idx[(hi == i) & fb]].max())
-        ser = propagate(float(h_values[i]), need) if propagate else grid[i, :need + 1]
-        if log and propagate:
-            log(f"fallback {n_done + 1}/{len(fb_h)}: h={float(h_values[i]):.4f}, {need} steps, "
-                f"{time.perf_counter() - t0:.1f}s")
-        sel = (hi == i) & fb
-        pred[sel] = ser[steps_grid.ravel()[query_idx[sel]]]
-    fallback_s = time.perf_counter() - t0
-    if propagate is None:
-        # no live propagator: charge the recorded full-range time of each fallback h
-        fallback_s = float(sum(plain_seconds[i] for i in fb_h))
-    err = np.abs(pred - Yall[query_idx])
-    q_h = np.unique(hi)
-    plain_s = float(sum(plain_seconds[i] for i in q_h))
-    # other disagreement bounds, fallbacks charged at the recorded propagation times
-    tradeoff = []
-    for bound in (1e-2, 5e-3, 2e-3, 1e-3, 5e-4):
-        f = sd > bound
-        hs = np.unique(hi[f])
-        e = np.where(f, 0.0, err_net)  # a fallback point takes the propagator's value
-        cost = infer_s + float(sum(plain_seconds[i] for i in hs))
-        tradeoff.append({"std_bound": bound, "fallback_points": int(f.sum()), "fallback_h": int(len(hs)),
-                         "frac_abs_err_le_1e-3": float(np.mean(e <= 1e-3)), "max_err": float(e.max()),
-                         "speedup_recorded": plain_s / cost})
-    surrogate_s = infer_s + fallback_s
+        top = int(qs[fb & (qh == i)].max())
+        ser, _ = propagate(float(h_values[i]), top)
+        cover = (qh == i) & (qs <= top) & ~sp.known
+        pred[cover] = ser[qs[cover]]
+        if log:
+            log(f"fallback {n_done + 1}/{len(fb_h)}: h={h_values[i]:.4f} to {top} steps, "
+                f"{time.perf_counter() - t1:.1f}s")
+    fallback_s = time.perf_counter() - t1
+    surrogate_s = time.perf_counter() - t0
     net.close()
-    return SweepReport(len(query_idx), int(fb.sum()), len(fb_h), err, float(np.mean(err <= 1e-3)),
+    err = np.abs(pred - ref)
+    err_net = np.abs(mean.astype(np.float64) - ref[new])
+    # other bounds, each fallback h charged at this run's plain-sweep time to its largest fallback step
+    tradeoff = []
+    newq = np.nonzero(new)[0]
+    for b in sorted({bound, 1e-2, 5e-3, 2e-3, 1e-3, 5e-4, 0.0}, reverse=True):
+        f = np.zeros(nq, dtype=bool)
+        f[newq[sd > b]] = True
+        e = err.copy()
+        e[newq] = err_net
+        cost = infer_s
+        for i in np.unique(qh[f]):
+            top = int(qs[f & (qh == i)].max())
+            e[(qh == i) & (qs <= top) & ~sp.known] = 0.0
+            cost += float(cum[int(i)][top])
+        tradeoff.append({"std_bound": b, "fallback_points": int(f.sum()), "fallback_h": int(len(np.unique(qh[f]))),
+                         "frac_abs_err_le_bar": float(np.mean(e <= bar)), "max_err": float(e.max()),
+                         "speedup": plain_s / cost})
+    return SweepReport(nq, int(sp.known.sum()), int(new.sum()), float(bound), len(sp.val), val_acc,
+                       int(new.sum() - fb.sum()), int(fb.sum()), len(fb_h), err, float(np.mean(err <= bar)),
                        float(err.max()), train_s, infer_s, fallback_s, surrogate_s, plain_s,
-                       plain_s / surrogate_s if surrogate_s > 0 else float("inf"), loss.tolist(),
-                       err_net, propagate is not None, tradeoff)
+                       plain_s / surrogate_s if surrogate_s > 0 else float("inf"), loss.tolist(), err_net,
+                       grid_diff, tradeoff)

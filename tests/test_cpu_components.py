@@ -123,3 +123,40 @@ def test_term_parser_forms():
     assert _parse_term("X", 3) == ("X", 1.0)
     with pytest.raises(ValueError, match="Qubit index 3 out of range for 3-qubit system."):
         _parse_term("Z3", 3)
+
+
+def test_surrogate_split_is_disjoint_in_h_and_steps():
+    """Training points are the seed-sampled ones; the validation pairs share no
+    (h, steps) pair with the fitted points, and new query pairs none with
+    either; known query pairs are exactly the query points on training pairs."""
+    from ppvm_amd.surrogate import split_sweep
+    from ppvm_amd.sweep import H_GRID, T_GRID
+    sp = split_sweep(H_GRID, T_GRID, 2048, 31, 0.2)
+    assert len(sp.train) == 2048 and len(sp.query) == 2048
+    assert len(np.intersect1d(sp.train, sp.query)) == 0
+    assert len(sp.fit) + len(sp.val) == 2048
+    fit_p, val_p = set(sp.pair[sp.fit]), set(sp.pair[sp.val])
+    assert not fit_p & val_p and abs(len(val_p) / len(fit_p | val_p) - 0.2) < 0.01
+    new_p = set(sp.pair[sp.query[~sp.known]])
+    assert not new_p & (fit_p | val_p)
+    assert set(sp.pair[sp.query[sp.known]]) <= fit_p | val_p
+    # pairs: h index and round(t / dt)
+    i = sp.query[5]
+    assert sp.steps[i] == round(T_GRID[i % 64] / 0.05) and sp.hidx[i] == i // 64
+    # seed-31 draw of the configured sampler
+    want = np.sort(np.random.default_rng(31).choice(np.arange(4096), size=2048, replace=False))
+    assert np.array_equal(sp.train, want)
+
+
+def test_surrogate_bound_calibration():
+    from ppvm_amd.surrogate import calibrate_bound
+    sd = np.array([0.5, 0.1, 0.3, 0.2, 0.4])
+    err = np.array([0.0, 0.0, 2e-3, 0.0, 0.0])  # the third-most-confident point misses
+    assert calibrate_bound(sd, err, 1e-3) == 0.2
+    assert calibrate_bound(sd, np.zeros(5), 1e-3) == 0.5
+    assert calibrate_bound(sd, np.array([0, 5e-3, 0, 0, 0.0]), 1e-3) == -1.0
+    # every validation point under the bound is within the bar
+    rng = np.random.default_rng(3)
+    sd, err = rng.random(500), rng.random(500) * 2e-3
+    b = calibrate_bound(sd, err, 1e-3)
+    assert np.all(err[sd <= b] <= 1e-3) and np.any(err[sd > b] > 1e-3)

@@ -1,15 +1,22 @@
-"""What the learned surrogate gains over the plain propagator sweep.
+"""What the learned surrogate gains over the plain propagator sweep, timed live.
 
-The ensemble is trained on the GPU on 2048 seed-31 points of the shipped
-propagator grid (10x10 lattice TFIM, 64 h x 64 t) and queried on the other
-2048.  Cost of the plain method for the query set: one propagation per h value
-that holds a query point (prefix reuse), at the wall times recorded with the
-grid.  Cost of the surrogate: inference plus one propagation per h value where
-the ensemble std exceeds 1e-2.  The test checks the gain (>= 10x) and the
-error distribution against the propagator values.
+The sweep here is the lattice TFIM grid of the surrogate workload restricted
+to t <= 1 (64 h x 32 t, steps <= 20), so that the plain sweep fits a test:
+the ensemble is fitted on the GPU on 1024 seed-31 training points of the
+shipped grid less a validation split of 20% of their (h, steps) pairs, the
+fallback bound on the ensemble std is calibrated on that split, and the other
+1024 points are queried (known pairs by lookup, new pairs by the ensemble
+under the bound, the propagator above it).  The plain sweep of the same query
+points runs live in the same test; its values judge every error and must
+reproduce the shipped grid exactly.
+
+Measured on one MI355X: the calibrated bound sends every new pair to the
+propagator (the ensemble agrees on points where the propagator's output jumps
+by more than 1e-3 between neighbouring h values), so at |err| <= 1e-3 the
+query phase gains only what the known pairs save (1.1x).  At the fixed std
+bound of 1e-2 the gain is 20x, with 88% of the points within 1e-3 and the
+rest within 1e-2.
 """
-
-import time
 
 import numpy as np
 import pytest
@@ -17,16 +24,33 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_surrogate_gain_over_plain_sweep():
-    from ppvm_amd.surrogate import load_grid, run_config5
+def _propagate(h, m):
+    from ppvm_amd import sweep as SW
+    ser, _, _, secs = SW.lattice_series(h, m)
+    return ser, secs
+
+
+@pytest.mark.timeout(600)
+def test_surrogate_sweep_against_live_plain_sweep():
+    from ppvm_amd.surrogate import load_grid, run_sweep
     d = load_grid()
-    t0 = time.perf_counter()
-    rep = run_config5(d["grid"], d["h"], d["t"], d["seconds"], steps=12000, lr=2e-3)
-    print(f"train {rep.train_s:.2f}s infer {rep.infer_s * 1e3:.2f} ms fallback h {rep.fallback_h} "
-          f"frac<=1e-3 {rep.frac_within_1e3:.3f} median {np.median(rep.abs_err):.2e} "
-          f"max {rep.max_err:.2e} speedup {rep.speedup:.1f} ({time.perf_counter() - t0:.1f}s)")
-    assert rep.n_query == 2048
-    assert rep.speedup >= 10.0
-    assert np.median(rep.abs_err) < 2e-3
-    assert rep.frac_within_1e3 > 0.4
-    assert rep.max_err < 0.1
+    tsel = d["t"] <= 1.0
+    rep = run_sweep(d["grid"], d["h"], d["t"][tsel], _propagate, n_train=1024, steps=12000, lr=2e-3,
+                    log=lambda m: print(m, flush=True))
+    print(f"bound {rep.bound:.2e}: known {rep.n_known} network {rep.n_accepted} fallback {rep.n_fallback} "
+          f"({rep.fallback_h} h); frac<=1e-3 {rep.frac_within_bar:.4f} max {rep.max_err:.2e}; "
+          f"query {rep.surrogate_s:.2f}s vs plain {rep.plain_s:.2f}s = {rep.speedup:.2f}x; "
+          f"validation {rep.val_accepted}/{rep.val_points}", flush=True)
+    for row in rep.tradeoff:
+        print(row, flush=True)
+    assert rep.n_query == 1024 and rep.n_known + rep.n_new == 1024
+    assert rep.grid_max_diff == 0.0  # the training grid is this propagator's own output
+    assert rep.n_accepted + rep.n_fallback == rep.n_new
+    # the calibrated bound keeps the bar on the query points and is not slower than the plain sweep
+    assert rep.frac_within_bar >= 0.99
+    assert rep.speedup >= 0.9
+    # what the network gains at the fixed std bound of 1e-2 (fallbacks charged at this run's
+    # plain-sweep times to the same step)
+    row = next(r for r in rep.tradeoff if r["std_bound"] == 1e-2)
+    assert row["speedup"] >= 10.0
+    assert row["frac_abs_err_le_bar"] >= 0.8 and row["max_err"] <= 1e-2
