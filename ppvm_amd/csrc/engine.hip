@@ -54,7 +54,7 @@ constexpr int kMaxGrid = 1024;
 constexpr int kOverlapGrid = 512;
 constexpr u32 kGenLimit = (1u << 24) - 2;
 
-enum IssuedKind { IK_ROT = 1, IK_MAP = 2, IK_SKIP = 3 };
+enum IssuedKind { IK_ROT = 1, IK_MAP = 2, IK_SKIP = 3, IK_PRES = 4 };
 struct Issued {
   int kind;
   u64 seq;
@@ -134,6 +134,7 @@ struct ppvm_sum {
   DevState* st = nullptr;
   DevState* st_host = nullptr;  // pinned
   u64* preserve = nullptr;
+  u64* pres_found = nullptr;  // k_pres_find's list: count, then up to n_preserve arena indices
   u32 n_preserve = 0;
   std::vector<u64> preserve_host;
   u64 op_seq = 0;
@@ -457,6 +458,24 @@ int launch_map(ppvm_sum* s, const Issued& it) {
   return PPVM_OK;
 }
 
+// Preserved entries that the truncation just applied would have dropped move
+// to the end of the term order (k_pres_find, k_pres_move).
+template <int W>
+int launch_pres(ppvm_sum* s, const Issued& it) {
+  const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
+  int rc;
+  const u32 G = (u32)grid_for(grid_entries(s), kBlock * 4, kFixedGrid);
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, s->pres_found, 1ull, 0ull);
+  hipLaunchKernelGGL(k_pres_find<W>, dim3(G), dim3(kBlock), 0, s->stream, s->keys[s->cur], s->coef[s->cur],
+                     s->st, in, s->trunc_spec(), s->pres_found);
+  hipLaunchKernelGGL(k_pres_move<W>, dim3(1), dim3(kPresMoveThreads), 0, s->stream, s->keys[s->cur],
+                     s->coef[s->cur], s->st, in, out, s->cap, s->n_preserve, s->pres_found, it.seq + 1);
+  if ((rc = prof_end(s))) return rc;
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
 template <int W>
 int launch_overlap(ppvm_sum* s) {
   int rc;
@@ -548,6 +567,7 @@ int launch_compact(ppvm_sum* s) {
 
 int issue_rot_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_rot, it); }
 int issue_map_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_map, it); }
+int issue_pres_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_pres, it); }
 int overlap_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_overlap); }
 int compact_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_compact); }
 int overlap2_w(ppvm_sum* s, ppvm_sum* b) { PPVM_DISPATCH_W(s, launch_overlap2, b); }
@@ -647,13 +667,18 @@ int launch_skip(ppvm_sum* s, const Issued& it) {
 }
 
 int issue(ppvm_sum* s, Issued& it, bool log_it) {
-  int rc = it.kind == IK_ROT ? issue_rot_w(s, it) : it.kind == IK_MAP ? issue_map_w(s, it) : launch_skip(s, it);
+  int rc = it.kind == IK_ROT    ? issue_rot_w(s, it)
+           : it.kind == IK_MAP  ? issue_map_w(s, it)
+           : it.kind == IK_PRES ? issue_pres_w(s, it)
+                                : launch_skip(s, it);
   if (rc) {
     s->broken = true;
     return rc;
   }
   if (it.kind == IK_ROT) {
     s->n_bound = std::min<u64>(s->cap, s->n_bound * 2 + 1);
+  } else if (it.kind == IK_PRES) {
+    s->n_bound = std::min<u64>(s->cap, s->n_bound + s->n_preserve);
   }
   if (log_it) s->log.push_back(it);
   return PPVM_OK;
@@ -976,6 +1001,15 @@ void sup_after_map(ppvm_sum* s, const ppvm_op& op) {
   }
 }
 
+// A truncation with a preserve set is followed by the op that moves the
+// preserved entries it would drop to the end.
+bool pres_active(const ppvm_sum* s) { return s->n_preserve > 0 && s->trunc_active; }
+int issue_pres(ppvm_sum* s) {
+  Issued it{};
+  it.kind = IK_PRES;
+  return issue_and_maybe_sync(s, it);
+}
+
 // Issues the pending run of dropped rotations (their bookkeeping kernel).
 int flush_skips(ppvm_sum* s) {
   if (!s->skip_run) return PPVM_OK;
@@ -1019,6 +1053,7 @@ int pump(ppvm_sum* s, bool final) {
       }
       if ((rc = flush_skips(s))) return rc;
       if ((rc = issue_and_maybe_sync(s, it))) return rc;
+      if (fused && pres_active(s) && (rc = issue_pres(s))) return rc;
       sup_after_rot(s, it.rot);
       i += fused ? 2 : 1;
       if (!fused && s->trunc_active) s->clean = false;
@@ -1051,12 +1086,17 @@ int pump(ppvm_sum* s, bool final) {
     size_t k = i;
     bool clean = s->clean || !s->trunc_active;
     bool any = false;
+    // with a preserve set, each truncation ends its program: the preserved
+    // entries it would drop move to the end before the next op runs
+    const bool pres = pres_active(s);
+    bool event = false;
     while (k < j && q[k].code == PPVM_OP_TRUNCATE) {
       if (s->trunc_active && !clean) p.trunc_before = 1;
+      event = event || p.trunc_before;
       clean = true;
       ++k;
     }
-    while (k < j) {
+    while (k < j && !(pres && event)) {
       const ppvm_op& m = q[k];
       if (m.code == PPVM_OP_NOP) {
         ++k;
@@ -1069,6 +1109,7 @@ int pump(ppvm_sum* s, bool final) {
       ++k;
       if (tnext) {
         clean = true;
+        event = true;
         while (k < j && q[k].code == PPVM_OP_TRUNCATE) ++k;
       } else if (s->trunc_active && !keeps_clean(m.code)) {
         clean = false;
@@ -1077,6 +1118,7 @@ int pump(ppvm_sum* s, bool final) {
     if (any || p.trunc_before) {
       if ((rc = flush_skips(s))) return rc;
       if ((rc = issue_and_maybe_sync(s, it))) return rc;
+      if (pres && event && (rc = issue_pres(s))) return rc;
     }
     s->clean = clean;
     i = k;
@@ -1301,6 +1343,7 @@ int ppvm_destroy(ppvm_handle h) {
   if (h->parts) (void)hipFree(h->parts);
   if (h->pmeta) (void)hipFree(h->pmeta);
   if (h->preserve) (void)hipFree(h->preserve);
+  if (h->pres_found) (void)hipFree(h->pres_found);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1315,12 +1358,16 @@ int ppvm_words(ppvm_handle h, uint32_t* w) {
 
 static int upload_preserve(ppvm_handle h) {
   if (h->preserve) (void)hipFree(h->preserve);
+  if (h->pres_found) (void)hipFree(h->pres_found);
   h->preserve = nullptr;
+  h->pres_found = nullptr;
   h->n_preserve = 0;
   const size_t cnt = h->preserve_host.size() / (2 * h->W);
   if (cnt) {
+    if (cnt >= (1ull << 32)) return fail(PPVM_EINVAL, "preserve set too large");
     HIPCHK(hipMalloc(&h->preserve, h->preserve_host.size() * 8));
     HIPCHK(hipMemcpy(h->preserve, h->preserve_host.data(), h->preserve_host.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMalloc(&h->pres_found, (cnt + 1) * 8));
     h->n_preserve = (u32)cnt;
   }
   return PPVM_OK;
@@ -1330,8 +1377,9 @@ int ppvm_set_preserve(ppvm_handle h, const uint64_t* x, const uint64_t* z, size_
   if (!h) return fail(PPVM_EINVAL, "null handle");
   if (count && (!x || !z)) return fail(PPVM_EINVAL, "null key array");
   int rc;
+  // ops issued so far (and any replay of them) keep the old set
   if ((rc = pump(h, true))) return rc;
-  HIPCHK(hipStreamSynchronize(h->stream));
+  if ((rc = sync_check(h))) return rc;
   const u32 W = h->W, Wa = h->Wa;
   h->preserve_host.assign((size_t)count * 2 * W, 0);
   for (size_t i = 0; i < count; ++i)

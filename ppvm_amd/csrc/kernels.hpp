@@ -1217,6 +1217,70 @@ __global__ void k_skip_gates(DevState* st, u32 in_slot, u32 out_slot, u32 n_gate
   st->rotations += n_gates;
 }
 
+// ---- preserved strings that a truncation would drop ---------------------------------
+// The reference's truncation removes every entry its strategy drops and then
+// re-inserts the preserved ones among them, so those leave their place and
+// follow every other entry, in their previous order (sum/data.rs:283-310).
+// Truncation here keeps such entries where they are; after each truncation
+// k_pres_find lists them (found[0] = count, found[1..] = arena indices, the
+// host clears the count first) and k_pres_move appends a copy of each at the
+// end of the arena, in index order, and marks the original dead.
+constexpr u32 kPresMoveThreads = 256;
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void k_pres_find(const u64* __restrict__ keys,
+                                                      const double* __restrict__ coef,
+                                                      const DevState* __restrict__ st, u32 in_slot,
+                                                      TruncSpec t, u64* __restrict__ found) {
+  if (st->overflow) return;
+  const u64 n = st->slot[in_slot].n_entries;
+  const bool wcap = t.maxw != ~0ull;
+  const u64 stride = (u64)gridDim.x * kBlock;
+  for (u64 i = (u64)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double c = coef[i];
+    if (is_dead(c)) continue;
+    const bool small = fabs(c) < t.tau;
+    if (!small && !wcap) continue;
+    u64 k[2 * W];
+    akey_load<W>(keys, i, k);
+    if (!small && weight_of<W>(k) <= t.maxw) continue;
+    if (!is_preserved<W>(k, t)) continue;
+    const u64 slot = atomicAdd((unsigned long long*)found, 1ull);
+    if (slot < t.n_preserve) found[1 + slot] = i;  // live keys are distinct: slot < n_preserve
+  }
+}
+
+template <int W>
+__global__ __launch_bounds__(kPresMoveThreads) void k_pres_move(u64* __restrict__ keys, double* __restrict__ coef,
+                                                                DevState* st, u32 in_slot, u32 out_slot, u64 cap,
+                                                                u32 n_preserve, const u64* __restrict__ found,
+                                                                u64 op_tag) {
+  if (st->overflow) return;
+  const CountSlot in = st->slot[in_slot];
+  const u64 m = found[0] < (u64)n_preserve ? found[0] : (u64)n_preserve;
+  if (in.n_entries + m > cap) {  // nothing written: the host grows the arena and replays this op
+    if (threadIdx.x == 0) cas_u64(&st->overflow, 0ull, op_tag);
+    return;
+  }
+  const u64* idx = found + 1;
+  for (u64 j = threadIdx.x; j < m; j += kPresMoveThreads) {
+    const u64 src = idx[j];
+    u64 r = 0;  // rank of src among the listed indices
+    for (u64 q = 0; q < m; ++q) r += idx[q] < src ? 1u : 0u;
+    u64 k[2 * W];
+    akey_load<W>(keys, src, k);
+    akey_store<W>(keys, in.n_entries + r, k);
+    coef[in.n_entries + r] = coef[src];
+    coef[src] = dead_value();
+  }
+  if (threadIdx.x == 0) {
+    CountSlot o = in;
+    o.n_entries = in.n_entries + m;
+    o.n_dead = in.n_dead + m;
+    st->slot[out_slot] = o;
+  }
+}
+
 // ---- k_finish: single-block reduction / bookkeeping ---------------------------
 enum FinishMode : u32 { F_MAP = 0, F_OVERLAP = 1 };
 struct FinishArgs {

@@ -147,6 +147,76 @@ def test_preserve_strings_kept():
     assert dict(g.terms) == {"ZIII": 1.0}
 
 
+def _low_weight_terms(rng, n, act, count):
+    """count distinct strings of weight 1-3 on the last act of n qubits."""
+    out = set()
+    while len(out) < count:
+        w = ["I"] * n
+        for q in rng.choice(act, size=int(rng.integers(1, 4)), replace=False):
+            w[n - act + int(q)] = "XYZ"[int(rng.integers(0, 3))]
+        out.add("".join(w))
+    return sorted(out)
+
+
+def _dropped_keys(n, strings, coeffs, tau, maxw, prog, count):
+    """Keys that a run of ``prog`` without a preserve set truncates away, in the
+    order they first vanish (the oracle, op by op)."""
+    from oracle.pyoracle import OracleSum
+    r = OracleSum(n, tau, maxw)
+    r.add_strings(strings, coeffs)
+    prev, out = set(strings), []
+    for i in range(len(prog)):
+        r.apply(prog[i:i + 1])
+        cur = {k for k, _ in r.terms()}
+        out += [k for k in sorted(prev - cur) if k not in out]
+        prev = cur
+        if len(out) >= count:
+            break
+    return out[:count]
+
+
+@pytest.mark.parametrize("n,tau,maxw,every,cap", [(6, 2e-2, None, 1, 0), (40, 1e-2, 5, 1, 0),
+                                                  (70, 5e-3, 6, 5, 0), (40, 1e-2, 5, 1, 1)])
+def test_preserved_strings_follow_the_reference_order(n, tau, maxw, every, cap):
+    """A preserved string that a truncation would drop leaves its place and
+    follows every other term, after every truncation (sum/data.rs:283-310):
+    through rotations, Cliffords under a weight cap, truncation every op or
+    every fifth op, and arena growth with replay (cap 1)."""
+    from oracle.pyoracle import OracleSum
+    from ppvm_amd import PauliSum
+    rng = np.random.default_rng(40 + n + cap)
+    act = min(n, 8)  # the program acts on the last 8 qubits (word 0 and, for n > 64, word 1)
+    strings = _low_weight_terms(rng, n, act, 12)
+    coeffs = rng.normal(size=12)
+    raw = random_program(rng, act, 150, trunc_every=False)
+    raw["q0"] += n - act
+    two = np.isin(raw["code"], list(O.TWO_QUBIT))
+    raw["q1"][two] += n - act
+    p = O.Program()
+    for i, op in enumerate(raw):
+        p.add(int(op["code"]), int(op["q0"]), int(op["q1"]), float(op["theta"]))
+        if i % every == every - 1:
+            p.truncate()
+    prog = p.array()
+    keep = _dropped_keys(n, strings, coeffs, tau, maxw, prog, 8)
+    assert len(keep) >= 4
+    g = PauliSum(strings, n, coefficients=coeffs, min_abs_coeff=tau, max_pauli_weight=maxw,
+                 preserve_strings=keep, capacity_hint=cap)
+    r = OracleSum(n, tau, maxw)
+    r.add_strings(strings, coeffs)
+    r.set_preserve(keep)
+    rescued, prev = 0, 0
+    for m in list(range(2 * every - 1, len(prog), 12)) + [len(prog) - 1]:
+        g.apply(prog[prev:m + 1])
+        r.apply(prog[prev:m + 1])
+        prev = m + 1
+        assert_same_terms(g._export(), r.export(), True, f"preserve n={n} op {m}")
+        # preserved entries present only because they are preserved
+        rescued += sum(1 for k, c in r.terms() if k in keep and
+                       (abs(c) < tau or (maxw is not None and sum(ch != "I" for ch in k) > maxw)))
+    assert rescued >= 2  # the set is exercised
+
+
 def test_clone_is_deep():
     w = WL.truncate_steps(WL.chain_tfim(), 2)
     g = gpu_for(w)
