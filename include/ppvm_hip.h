@@ -229,8 +229,9 @@ int ppvm_get_stats(ppvm_handle h, ppvm_stats* out);
  * launches, bit-sliced planes read by scans (entries x planes), capacity,
  * syncs, regrows, compactions, peak live terms, entries transposed into the
  * bit-sliced planes, worklist records that the partitioned pairing sent to
- * the global table (buckets too large for one workgroup's LDS). */
-int ppvm_get_counters(ppvm_handle h, uint64_t out[18]);
+ * the global table (buckets too large for one workgroup's LDS), LDS-resident
+ * runs of a small sum, and the steps those runs completed. */
+int ppvm_get_counters(ppvm_handle h, uint64_t out[20]);
 
 /* Per-phase kernel timing (HIP events on the handle's stream), for the
  * measurement harness.  Phases: 0 rotation scan, 1 rotation pair/merge,

@@ -275,10 +275,11 @@ class Handle:
 
     COUNTERS = ("term_updates", "gates", "rotations", "scanned", "wl_total", "pairs_total",
                 "children_total", "kills_total", "map_entries", "map_launches", "scan_planes",
-                "capacity", "syncs", "regrows", "compactions", "peak_live", "transposed", "pair_fallback")
+                "capacity", "syncs", "regrows", "compactions", "peak_live", "transposed", "pair_fallback",
+                "small_runs", "small_steps")
 
     def counters(self) -> dict:
-        out = np.zeros(18, dtype=np.uint64)
+        out = np.zeros(len(self.COUNTERS), dtype=np.uint64)
         check(lib().ppvm_get_counters(self.h, u64p(out)))
         return {k: int(v) for k, v in zip(self.COUNTERS, out)}
 

@@ -70,7 +70,9 @@ struct alignas(64) DevState {
   u64 tvw;            // words of the bit-sliced planes that are current (entries < 64 tvw)
   u64 transposed;     // entries written into the bit-sliced planes
   u64 pair_fallback;  // records the partitioned pairing sent to the global table
-  u64 pad[4];
+  u64 small_done;     // steps completed by the last LDS-resident run (k_small)
+  u64 small_peak;     // largest live count inside that run
+  u64 pad[2];
 };
 
 // Branching op: a rotation exp(-i theta/2 G) with G acting on qubits a (and

@@ -24,6 +24,7 @@
 
 #include "../../include/ppvm_hip.h"
 #include "kernels.hpp"
+#include "small.hpp"
 
 using namespace ppvm;
 
@@ -55,6 +56,13 @@ constexpr int kOverlapGrid = 512;
 constexpr u32 kGenLimit = (1u << 24) - 2;
 
 enum IssuedKind { IK_ROT = 1, IK_MAP = 2, IK_SKIP = 3, IK_PRES = 4 };
+
+// LDS-resident runs (k_small): steps collected per run, and the map programs among them.
+constexpr u32 kSmallMaxSteps = 4096;
+constexpr u32 kSmallMaxMaps = 512;
+constexpr u32 kSmallThreads = 512;
+constexpr size_t kSmallBufferBytes =
+    (size_t)kSmallMaxSteps * sizeof(SmallStep) + (size_t)kSmallMaxMaps * sizeof(MapProgram);
 struct Issued {
   int kind;
   u64 seq;
@@ -145,6 +153,7 @@ struct ppvm_sum {
   u32 tab_gen = 0;
   bool clean = true;
   bool broken = false;
+  bool fresh = false;  // st_host holds the state after the last issued op, and sync_check's work is done
   std::vector<ppvm_op> pending;
   std::vector<Issued> log;
   int issued_since_sync = 0;
@@ -153,6 +162,19 @@ struct ppvm_sum {
   u64 partition_min = 0;  // pair through the radix partition from this many entries on
   u32 bucket_lg_max = 0;  // at most 2^bucket_lg_max buckets in the partition
   ppvm_stats hs{};
+  // LDS-resident runs of small sums: while small_on, the planner collects its
+  // steps in small_list instead of issuing them, and small_flush runs them in
+  // one k_small launch.
+  bool small_on = false;
+  bool small_off = false;   // off for this handle (PPVM_SMALL_MAX_ENTRIES=0, or a failed launch)
+  u64 small_max = 0;        // collect while the synchronised arena holds at most this many entries
+  u32 small_threads = kSmallThreads;
+  std::vector<Issued> small_list;
+  u32 small_maps = 0;       // map programs in small_list
+  // step records, then map programs, in one device buffer (filled from pinned staging by one copy)
+  unsigned char* d_small = nullptr;
+  unsigned char* h_small = nullptr;
+  u64 small_runs = 0, small_steps = 0;
   // profiling
   bool prof = false;
   std::vector<hipEvent_t> ev;
@@ -358,6 +380,29 @@ u32 bucket_lg_limit() {
   return (end && *end == '\0' && v < kMaxBucketsLg) ? (u32)v : kMaxBucketsLg;
 }
 
+// Sums whose synchronised arena holds at most this many entries run their
+// pending steps in LDS (k_small): three quarters of the LDS capacity by
+// default, so a run has room to grow (a run that outgrows LDS hands over to
+// the arena pipeline); PPVM_SMALL_MAX_ENTRIES overrides it (0 turns the path off;
+// both paths give identical results).  PPVM_SMALL_THREADS sets the
+// workgroup size (a multiple of 64 up to 1024).
+u64 small_cap(u32 W);
+u64 small_max_entries(u32 W) {
+  const u64 dflt = small_cap(W) / 4 * 3;
+  const char* e = std::getenv("PPVM_SMALL_MAX_ENTRIES");
+  if (!e || !*e) return dflt;
+  char* end = nullptr;
+  const unsigned long long v = std::strtoull(e, &end, 10);
+  return (end && *end == '\0') ? std::min<u64>((u64)v, small_cap(W)) : dflt;
+}
+u32 small_threads_env() {
+  const char* e = std::getenv("PPVM_SMALL_THREADS");
+  if (!e || !*e) return kSmallThreads;
+  char* end = nullptr;
+  const unsigned long v = std::strtoul(e, &end, 10);
+  return (end && *end == '\0' && v >= 64 && v <= kSmallMaxThreads && v % 64 == 0) ? (u32)v : kSmallThreads;
+}
+
 // Arena size the launch grids of rotations and map programs are sized for.
 // Those kernels loop over chunks (or grid-stride), so a grid smaller than the
 // arena is only slower, never wrong; the host's hard bound doubles with every
@@ -476,6 +521,28 @@ int launch_pres(ppvm_sum* s, const Issued& it) {
   return PPVM_OK;
 }
 
+// One LDS-resident run of the collected steps (k_small).
+template <int W>
+int launch_small(ppvm_sum* s, const Issued& it, u32 n_steps) {
+  const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
+  SmallArgs a{};
+  a.keys = s->keys[s->cur];
+  a.coef = s->coef[s->cur];
+  a.st = s->st;
+  a.in_slot = in;
+  a.out_slot = out;
+  a.steps = reinterpret_cast<const SmallStep*>(s->d_small);
+  a.n_steps = n_steps;
+  a.maps = reinterpret_cast<const MapProgram*>(s->d_small + (size_t)n_steps * sizeof(SmallStep));
+  a.t = s->trunc_spec();
+  int rc;
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_small<W>, dim3(1), dim3(s->small_threads), 0, s->stream, a);
+  if ((rc = prof_end(s))) return rc;
+  HIPCHK(hipGetLastError());
+  return PPVM_OK;
+}
+
 template <int W>
 int launch_overlap(ppvm_sum* s) {
   int rc;
@@ -568,6 +635,7 @@ int launch_compact(ppvm_sum* s) {
 int issue_rot_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_rot, it); }
 int issue_map_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_map, it); }
 int issue_pres_w(ppvm_sum* s, const Issued& it) { PPVM_DISPATCH_W(s, launch_pres, it); }
+int small_w(ppvm_sum* s, const Issued& it, u32 n) { PPVM_DISPATCH_W(s, launch_small, it, n); }
 int overlap_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_overlap); }
 int compact_w(ppvm_sum* s) { PPVM_DISPATCH_W(s, launch_compact); }
 int overlap2_w(ppvm_sum* s, ppvm_sum* b) { PPVM_DISPATCH_W(s, launch_overlap2, b); }
@@ -667,6 +735,7 @@ int launch_skip(ppvm_sum* s, const Issued& it) {
 }
 
 int issue(ppvm_sum* s, Issued& it, bool log_it) {
+  s->fresh = false;
   int rc = it.kind == IK_ROT    ? issue_rot_w(s, it)
            : it.kind == IK_MAP  ? issue_map_w(s, it)
            : it.kind == IK_PRES ? issue_pres_w(s, it)
@@ -744,6 +813,7 @@ int launch_fingerprint(ppvm_sum* s, u32 G) {
 
 int sync_check(ppvm_sum* s) {
   if (s->broken) return fail(PPVM_ESTATE, "handle is unusable after an earlier device error");
+  if (s->fresh) return PPVM_OK;
   int rc;
   for (int round = 0; round < 64; ++round) {
     if ((rc = read_state(s))) {
@@ -811,6 +881,7 @@ int sync_check(ppvm_sum* s) {
     s->broken = true;
     return rc;
   }
+  s->fresh = true;
   return PPVM_OK;
 }
 
@@ -913,7 +984,22 @@ bool map_add(MapProgram& p, u32& ntables, const ppvm_op& op, bool trunc) {
   return true;
 }
 
+int small_flush(ppvm_sum* s);
+bool small_eligible(const ppvm_sum* s);
+
 int issue_and_maybe_sync(ppvm_sum* s, Issued& it) {
+  if (s->small_on) {
+    if (s->small_list.size() >= kSmallMaxSteps || (it.kind == IK_MAP && s->small_maps >= kSmallMaxMaps)) {
+      int rc = small_flush(s);  // full: run what is collected, then go on collecting if still small
+      if (rc) return rc;
+      s->small_on = small_eligible(s);
+    }
+    if (s->small_on && it.kind != IK_PRES) {
+      s->small_maps += it.kind == IK_MAP ? 1u : 0u;
+      s->small_list.push_back(it);
+      return PPVM_OK;
+    }
+  }
   it.seq = s->op_seq++;
   int rc = issue(s, it, true);
   if (rc) return rc;
@@ -1020,11 +1106,123 @@ int flush_skips(ppvm_sum* s) {
   return issue_and_maybe_sync(s, it);
 }
 
-// Turns queued ops into kernel launches.  With final=false, a trailing op whose
+// ---- LDS-resident runs of small sums ------------------------------------------------
+u64 small_cap(u32 W) {
+  return W == 1 ? SmallCap<1>::cap : W == 2 ? SmallCap<2>::cap : W == 4 ? SmallCap<4>::cap : SmallCap<8>::cap;
+}
+
+// A pump collects steps for k_small when the state is synchronised (nothing
+// issued since, so the run starts from known counts), has no preserve set, and
+// the arena is small.
+bool small_eligible(const ppvm_sum* s) {
+  return !s->small_off && !s->broken && s->n_preserve == 0 && s->log.empty() && s->issued_since_sync == 0 &&
+         s->n_synced <= s->small_max;
+}
+
+int small_buffers(ppvm_sum* s) {
+  if (s->d_small) return PPVM_OK;
+  if (hipMalloc(&s->d_small, kSmallBufferBytes) != hipSuccess ||
+      hipHostMalloc(&s->h_small, kSmallBufferBytes, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(PPVM_ENOMEM, "allocation of the small-run step buffers failed");
+  }
+  return PPVM_OK;
+}
+
+// Runs the collected steps in one k_small launch, synchronises, and issues the
+// steps it could not complete (the sum outgrew LDS) through the arena pipeline.
+int small_flush(ppvm_sum* s) {
+  if (s->small_list.empty()) return PPVM_OK;
+  std::vector<Issued> list;
+  list.swap(s->small_list);
+  s->small_maps = 0;
+  s->small_on = false;
+  int rc;
+  if ((rc = small_buffers(s))) return rc;
+  // step records, then the map programs right after them
+  SmallStep* hs = reinterpret_cast<SmallStep*>(s->h_small);
+  MapProgram* hm = reinterpret_cast<MapProgram*>(s->h_small + list.size() * sizeof(SmallStep));
+  u32 nm = 0;
+  for (size_t k = 0; k < list.size(); ++k) {
+    const Issued& it = list[k];
+    SmallStep& st = hs[k];
+    std::memset(&st, 0, sizeof(st));
+    if (it.kind == IK_ROT) {
+      st.kind = SK_ROT;
+      st.rot = it.rot;
+    } else if (it.kind == IK_MAP) {
+      st.kind = SK_MAP;
+      st.arg = nm;
+      hm[nm++] = it.map;
+    } else {
+      st.kind = SK_SKIP;
+      st.arg = it.nskip;
+    }
+  }
+  const u32 n = (u32)list.size();
+  const size_t bytes = (size_t)n * sizeof(SmallStep) + (size_t)nm * sizeof(MapProgram);
+  HIPCHK(hipMemcpyAsync(s->d_small, s->h_small, bytes, hipMemcpyHostToDevice, s->stream));
+  Issued run{};
+  run.seq = s->op_seq;
+  if ((rc = small_w(s, run, n))) {
+    // not launched: the slots are as they were; run everything through the arena pipeline
+    s->small_off = true;
+    for (auto& it : list)
+      if ((rc = issue_and_maybe_sync(s, it))) return rc;
+    return PPVM_OK;
+  }
+  s->op_seq += 1;
+  if ((rc = read_state(s))) {
+    s->broken = true;
+    return rc;
+  }
+  const u64 done = s->st_host->small_done;
+  const CountSlot& c = s->st_host->slot[s->op_seq & 1];
+  s->n_bound = c.n_entries;
+  s->n_synced = c.n_entries;
+  s->hs.peak_live = std::max<u64>(s->hs.peak_live, s->st_host->small_peak);
+  s->small_runs += 1;
+  s->small_steps += done;
+  if (done == list.size()) {
+    // synchronised with nothing pending: sync_check has nothing to add but the
+    // arena's room for a gate of children (the support stays the planner's)
+    s->log.clear();
+    s->issued_since_sync = 0;
+    if (c.n_entries * 3 + 1 > s->cap && (rc = resize(s, pow2_at_least(c.n_entries * 3 + 1), c.n_entries))) {
+      s->broken = true;
+      return rc;
+    }
+    s->fresh = true;
+    return PPVM_OK;
+  }
+  for (size_t k = done; k < list.size(); ++k)
+    if ((rc = issue_and_maybe_sync(s, list[k]))) return rc;
+  return PPVM_OK;
+}
+
+// Turns queued ops into kernel launches.// Turns queued ops into kernel launches.  With final=false, a trailing op whose
 // fusion partner may still arrive (a rotation that a truncate could follow, a
 // run of map ops) is kept queued.
+int pump_ops(ppvm_sum* s, bool final);
+
+// Collection for an LDS-resident run starts at a pump on a small synchronised
+// state and goes on over later pumps; the run is launched by the final pump
+// that every call reading or changing the state makes first (or earlier, when
+// the step list is full).
 int pump(ppvm_sum* s, bool final) {
   if (s->broken) return fail(PPVM_ESTATE, "handle is unusable after an earlier device error");
+  if (!s->small_on) {
+    s->small_list.clear();
+    s->small_maps = 0;
+    s->small_on = small_eligible(s);
+  }
+  int rc = pump_ops(s, final);
+  if (!rc && final) rc = small_flush(s);
+  if (rc || final) s->small_on = false;
+  return rc;
+}
+
+int pump_ops(ppvm_sum* s, bool final) {
   size_t i = 0;
   auto& q = s->pending;
   int rc = PPVM_OK;
@@ -1236,6 +1434,7 @@ int upload(ppvm_sum* s, const std::vector<u64>& keys, const std::vector<double>&
   CountSlot c{};
   c.n_entries = n;
   c.n_dead = 0;
+  s->fresh = false;
   HIPCHK(hipMemcpyAsync(&s->st->slot[s->op_seq & 1], &c, sizeof(CountSlot), hipMemcpyHostToDevice, s->stream));
   if ((rc = invalidate_tp(s))) return rc;
   HIPCHK(hipStreamSynchronize(s->stream));
@@ -1276,6 +1475,9 @@ int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint,
   s->trunc_active = tau > 0.0 || maxw != ~0ull;
   s->partition_min = partition_min_entries();
   s->bucket_lg_max = bucket_lg_limit();
+  s->small_max = small_max_entries(Wp);
+  s->small_off = s->small_max == 0;
+  s->small_threads = small_threads_env();
   s->xsup.assign(Wp, 0ull);
   s->zsup.assign(Wp, 0ull);
   s->clean = true;
@@ -1344,6 +1546,8 @@ int ppvm_destroy(ppvm_handle h) {
   if (h->pmeta) (void)hipFree(h->pmeta);
   if (h->preserve) (void)hipFree(h->preserve);
   if (h->pres_found) (void)hipFree(h->pres_found);
+  if (h->d_small) (void)hipFree(h->d_small);
+  if (h->h_small) (void)hipHostFree(h->h_small);
   for (auto e : h->ev) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
@@ -1475,6 +1679,9 @@ int ppvm_clear(ppvm_handle h) {
   h->skip_run = 0;
   h->log.clear();
   h->issued_since_sync = 0;
+  h->small_list.clear();
+  h->small_maps = 0;
+  h->small_on = false;
   hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, h->stream, &h->st->overflow, 1ull, 0ull);
   HIPCHK(hipGetLastError());
   const std::vector<u64> keys;
@@ -1677,17 +1884,17 @@ int ppvm_get_stats(ppvm_handle h, ppvm_stats* out) {
   return PPVM_OK;
 }
 
-int ppvm_get_counters(ppvm_handle h, uint64_t out[18]) {
+int ppvm_get_counters(ppvm_handle h, uint64_t out[20]) {
   if (!h || !out) return fail(PPVM_EINVAL, "null argument");
   int rc;
   if ((rc = ppvm_synchronize(h))) return rc;
   const DevState& d = *h->st_host;
-  const u64 v[18] = {d.term_updates, d.gates,          d.rotations,   d.scanned,
+  const u64 v[20] = {d.term_updates, d.gates,          d.rotations,   d.scanned,
                      d.wl_total,     d.pairs_total,    d.children_total, d.kills_total,
                      d.map_entries,  d.map_launches,   d.scan_planes, h->cap,
                      h->hs.syncs,    h->hs.regrows,    h->hs.compactions, h->hs.peak_live,
-                     d.transposed,   d.pair_fallback};
-  for (int i = 0; i < 18; ++i) out[i] = v[i];
+                     d.transposed,   d.pair_fallback,  h->small_runs, h->small_steps};
+  for (int i = 0; i < 20; ++i) out[i] = v[i];
   return PPVM_OK;
 }
 

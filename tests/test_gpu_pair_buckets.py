@@ -46,6 +46,7 @@ def test_single_bucket_branches(m, monkeypatch):
     from oracle.pyoracle import OracleSum
     from ppvm_amd import PauliSum
     monkeypatch.setenv("PPVM_PARTITION_MIN_ENTRIES", "0")
+    monkeypatch.setenv("PPVM_SMALL_MAX_ENTRIES", "0")
     monkeypatch.setenv("PPVM_PARTITION_MAX_LG", "0")
     rng = np.random.default_rng(m)
     x, z, c = _words(rng, m)

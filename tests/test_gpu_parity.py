@@ -17,13 +17,21 @@ from tests.helpers import (assert_same_terms, gpu_for, oracle_for, random_progra
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["global", "partition"])
+@pytest.fixture(autouse=True, params=["global", "partition", "small"])
 def pairing_path(request, monkeypatch):
-    """Run every test with both pairing implementations: the single
-    global-table kernel and the radix partition + LDS kernels (forced at every
-    size).  Handles read the choice when they are created."""
-    monkeypatch.setenv("PPVM_PARTITION_MIN_ENTRIES",
-                       "0" if request.param == "partition" else str(2**63))
+    """Run every test on each execution path: the arena pipeline with the
+    single global-table pairing kernel, the arena pipeline with the radix
+    partition + LDS pairing kernels (forced at every size), and the default
+    settings, under which a small sum runs its steps in LDS in one workgroup
+    (k_small) and hands over to the arena pipeline when it outgrows LDS.
+    Handles read the choice when they are created."""
+    if request.param == "small":
+        monkeypatch.delenv("PPVM_PARTITION_MIN_ENTRIES", raising=False)
+        monkeypatch.delenv("PPVM_SMALL_MAX_ENTRIES", raising=False)
+    else:
+        monkeypatch.setenv("PPVM_SMALL_MAX_ENTRIES", "0")
+        monkeypatch.setenv("PPVM_PARTITION_MIN_ENTRIES",
+                           "0" if request.param == "partition" else str(2**63))
     return request.param
 
 
