@@ -185,6 +185,44 @@ __device__ __forceinline__ bool small_item(const u64* P, const double* C, const 
   return true;
 }
 
+// Both items of entry i at once (one set of loads): the entry with its new
+// coefficient, and (candc) the child of an unpaired branching entry.
+template <int W, u32 CAP>
+__device__ __forceinline__ void small_items(const u64* P, const double* C, const double* D, const u32* partner,
+                                            u32 i, const RotSpec& g, const TruncSpec& t, u32 tag,
+                                            u64 (&k)[2 * W], double& c, bool& keep, u64 (&kc)[2 * W],
+                                            double& cc, bool& keepc, bool& candc) {
+  sload<W, CAP>(P, i, k);
+  const double c0 = C[i], dp = D[i];
+  const u32 pl = partner[i];
+  const bool anti = rot_anti<W>(k, g) != 0u;
+  const bool paired = anti && (pl & 0xffff0000u) == tag;
+  const bool branches = anti && (g.mode != kBranchDamping || zbit<W>(k, g.a));
+  c = paired ? dp : (branches ? __dmul_rn(c0, g.cs) : c0);
+  keep = !g.trunc || !drop_term<W>(k, c, t);
+  candc = branches && !paired;
+  keepc = false;
+  if (candc) {
+    cc = __dmul_rn(c0, rot_eps<W>(k, g) * g.sn);
+#pragma unroll
+    for (int p = 0; p < 2 * W; ++p) kc[p] = k[p];
+    rot_flip<W>(kc, g);
+    keepc = !g.trunc || !drop_term<W>(kc, cc, t);
+  }
+}
+
+// Slot of a pair key in the LDS table (TS a power of two): one 64-bit
+// multiply per word and a 32-bit finish, top bits.
+template <int W, u32 TS>
+__device__ __forceinline__ u32 small_slot(const u64 (&r)[2 * W]) {
+  u64 h = r[0];
+#pragma unroll
+  for (int p = 1; p < 2 * W; ++p) h = (h ^ (h >> 31)) * 0x9E3779B97F4A7C15ull + r[p];
+  h = (h ^ (h >> 29)) * 0xBF58476D1CE4E5B9ull;
+  const u32 x = (u32)(h >> 32) ^ (u32)h;
+  return (x * 0x85EBCA6Bu) >> (32 - __builtin_ctz(TS));
+}
+
 // One rotation or damping branch: P/C (n entries) -> Q/D.  Table slots and
 // partner links carry the generation gen in their upper 16 bits, so neither
 // needs clearing between rotations.  The member of a pair that finds the
@@ -210,7 +248,7 @@ __device__ __forceinline__ int small_rotate(const u64* P, const double* C, u64* 
     ++ty.wl;
     u64 rep[2 * W];
     pair_key<W>(k, g, rep);
-    u32 pos = (u32)hash_key<W>(rep) & (TS - 1u);
+    u32 pos = small_slot<W, TS>(rep);
     u32 v = table[pos];
     while (true) {
       if ((v & 0xffff0000u) != tag) {
@@ -250,7 +288,8 @@ __device__ __forceinline__ int small_rotate(const u64* P, const double* C, u64* 
       v = table[pos];
     }
   }
-  if (!__syncthreads_or(any ? 1 : 0)) return 0;
+  u32 unused;
+  if (small_rank(any, unused, wcnt, par) == 0) return 0;  // one barrier: does any entry anticommute?
   if (n <= T) {
     // one item per thread: the entry and its child ranked together
     const u32 i = tid;
@@ -258,8 +297,8 @@ __device__ __forceinline__ int small_rotate(const u64* P, const double* C, u64* 
     double c = 0.0, cc = 0.0;
     bool keep = false, keepc = false, cand = false, candc = false;
     if (i < n) {
-      cand = small_item<W, CAP>(P, C, D, partner, i, 0, g, t, tag, k, c, keep);
-      candc = small_item<W, CAP>(P, C, D, partner, i, 1, g, t, tag, kc, cc, keepc);
+      cand = true;
+      small_items<W, CAP>(P, C, D, partner, i, g, t, tag, k, c, keep, kc, cc, keepc, candc);
     }
     ty.kills += ((cand && !keep) ? 1u : 0u) + ((candc && !keepc) ? 1u : 0u);
     ty.children += keepc ? 1u : 0u;
