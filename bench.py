@@ -112,6 +112,10 @@ def measured_traffic(kernel: str):
     try:
         with open(TRAFFIC_FILE) as f:
             for line in f:
+                # the first table averages over every dispatch of the run; later
+                # tables (a subset of gates) are not per-launch averages of it
+                if rows and line.startswith("# HBM traffic"):
+                    break
                 parts = line.split()
                 if len(parts) == 5 and parts[0].startswith("k_"):
                     rows[parts[0]] = (int(parts[1]), float(parts[4]))

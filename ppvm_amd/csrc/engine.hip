@@ -66,7 +66,8 @@ constexpr size_t kSmallBufferBytes =
 struct Issued {
   int kind;
   u64 seq;
-  u32 nskip;  // IK_SKIP: rotations in the run
+  u32 nskip;     // IK_SKIP: rotations in the run
+  u32 mlo, mhi;  // IK_MAP collected for an LDS run: its source ops in small_mapops
   RotSpec rot;
   MapProgram map;
 };
@@ -171,6 +172,8 @@ struct ppvm_sum {
   u32 small_threads = kSmallThreads;
   std::vector<Issued> small_list;
   u32 small_maps = 0;       // map programs in small_list
+  std::vector<ppvm_op> small_mapops;  // source ops of the collected map programs
+  bool sup_stale = false;   // support is the planner's widened estimate after an LDS run
   // step records, then map programs, in one device buffer (filled from pinned staging by one copy)
   unsigned char* d_small = nullptr;
   unsigned char* h_small = nullptr;
@@ -1185,7 +1188,9 @@ int small_flush(ppvm_sum* s) {
   s->small_steps += done;
   if (done == list.size()) {
     // synchronised with nothing pending: sync_check has nothing to add but the
-    // arena's room for a gate of children (the support stays the planner's)
+    // arena's room for a gate of children; the support is the planner's
+    // estimate, widened over every collected step, and is read back exactly
+    // before the next ops are planned
     s->log.clear();
     s->issued_since_sync = 0;
     if (c.n_entries * 3 + 1 > s->cap && (rc = resize(s, pow2_at_least(c.n_entries * 3 + 1), c.n_entries))) {
@@ -1193,11 +1198,39 @@ int small_flush(ppvm_sum* s) {
       return rc;
     }
     s->fresh = true;
+    s->sup_stale = true;
     return PPVM_OK;
   }
-  for (size_t k = done; k < list.size(); ++k)
-    if ((rc = issue_and_maybe_sync(s, list[k]))) return rc;
-  return PPVM_OK;
+  // The sum outgrew LDS at step `done`.  The remaining steps were planned with
+  // the support widened over the whole collection, so rotations that act on no
+  // term of the actual state would be launched: decide again from the exact
+  // support of the state the run left, widened by the remaining steps as they
+  // are issued.  Dropping a rotation that meets no anticommuting term changes
+  // nothing (a truncation fused with it was planned on a clean state).
+  if ((rc = refresh_support(s, c.n_entries))) {
+    s->broken = true;
+    return rc;
+  }
+  s->sup_stale = false;
+  for (size_t k = done; k < list.size(); ++k) {
+    Issued& it = list[k];
+    if (it.kind == IK_SKIP) {
+      s->skip_run += it.nskip;
+      continue;
+    }
+    if (it.kind == IK_ROT && it.rot.mode != kBranchDamping && !rot_may_act(s, it.rot)) {
+      ++s->skip_run;
+      continue;
+    }
+    if ((rc = flush_skips(s))) return rc;
+    if ((rc = issue_and_maybe_sync(s, it))) return rc;
+    if (it.kind == IK_ROT) {
+      sup_after_rot(s, it.rot);
+    } else if (it.kind == IK_MAP) {
+      for (u32 j = it.mlo; j < it.mhi; ++j) sup_after_map(s, s->small_mapops[j]);
+    }
+  }
+  return flush_skips(s);
 }
 
 // Turns queued ops into kernel launches.// Turns queued ops into kernel launches.  With final=false, a trailing op whose
@@ -1214,7 +1247,21 @@ int pump(ppvm_sum* s, bool final) {
   if (!s->small_on) {
     s->small_list.clear();
     s->small_maps = 0;
+    s->small_mapops.clear();
     s->small_on = small_eligible(s);
+  }
+  if (s->sup_stale && !s->pending.empty() && s->log.empty() && s->issued_since_sync == 0) {
+    // an LDS run left the planner's estimate: plan the next ops from the exact
+    // support (the state is synchronised and small)
+    const u64 n = s->n_synced;
+    if (n <= kSupportRefreshMax) {
+      int rc = refresh_support(s, n);
+      if (rc) {
+        s->broken = true;
+        return rc;
+      }
+    }
+    s->sup_stale = false;
   }
   int rc = pump_ops(s, final);
   if (!rc && final) rc = small_flush(s);
@@ -1278,6 +1325,7 @@ int pump_ops(ppvm_sum* s, bool final) {
     }
     Issued it{};
     it.kind = IK_MAP;
+    it.mlo = it.mhi = (u32)s->small_mapops.size();
     MapProgram& p = it.map;
     std::memset(&p, 0, sizeof(p));
     u32 ntables = 0;
@@ -1303,6 +1351,7 @@ int pump_ops(ppvm_sum* s, bool final) {
       const bool tnext = k + 1 < q.size() && q[k + 1].code == PPVM_OP_TRUNCATE && s->trunc_active;
       if (!map_add(p, ntables, m, tnext)) break;  // program full: issue and continue
       sup_after_map(s, m);
+      if (s->small_on) s->small_mapops.push_back(m);
       any = true;
       ++k;
       if (tnext) {
@@ -1313,6 +1362,7 @@ int pump_ops(ppvm_sum* s, bool final) {
         clean = false;
       }
     }
+    it.mhi = (u32)s->small_mapops.size();
     if (any || p.trunc_before) {
       if ((rc = flush_skips(s))) return rc;
       if ((rc = issue_and_maybe_sync(s, it))) return rc;
@@ -1441,6 +1491,7 @@ int upload(ppvm_sum* s, const std::vector<u64>& keys, const std::vector<double>&
   s->n_bound = n;
   s->n_synced = s->n_bound;
   sup_from_keys(s, keys, n);
+  s->sup_stale = false;
   return PPVM_OK;
 }
 
@@ -1615,6 +1666,7 @@ int ppvm_clone(ppvm_handle h, ppvm_handle* out) {
   c->partition_min = h->partition_min;
   c->xsup = h->xsup;
   c->zsup = h->zsup;
+  c->sup_stale = h->sup_stale;
   c->bucket_lg_max = h->bucket_lg_max;
   c->clean = h->clean;
   c->preserve_host = h->preserve_host;

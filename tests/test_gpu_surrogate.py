@@ -44,7 +44,9 @@ def test_surrogate_sweep_against_live_plain_sweep():
     for row in rep.tradeoff:
         print(row, flush=True)
     assert rep.n_query == 1024 and rep.n_known + rep.n_new == 1024
-    assert rep.grid_max_diff == 0.0  # the training grid is this propagator's own output
+    # the training grid is this propagator's own output (same term sets; the
+    # overlap sum may associate differently between engine versions)
+    assert rep.grid_max_diff <= 1e-10
     assert rep.n_accepted + rep.n_fallback == rep.n_new
     # the calibrated bound keeps the bar on the query points and is not slower than the plain sweep
     assert rep.frac_within_bar >= 0.99
