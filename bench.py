@@ -47,11 +47,11 @@ def _make(w, cap: int = 0):
 
 
 # Timed phases of the engine (HIP events per launch); "pairing" is k_rot_pair
-# on small gates and k_pair_hist + k_pair_colscan + k_pair_scatter + k_pair_lds
-# on large ones.
+# on small gates and k_pair_hist + k_pair_colscan + k_pair_scatter_tiled +
+# k_pair_lds on large ones.
 KERNELS = ["k_rot_count", "k_rot_gather", "pairing", "k_emit_count", "k_emit_write", "k_map",
            "k_finish", "other"]
-PAIRING_KERNELS = ("k_rot_pair", "k_pair_hist", "k_pair_colscan", "k_pair_scatter", "k_pair_lds")
+PAIRING_KERNELS = ("k_rot_pair", "k_pair_hist", "k_pair_colscan", "k_pair_scatter_tiled", "k_pair_lds")
 
 
 def phase_bytes(cnt: dict, W: int, nq: int) -> dict:

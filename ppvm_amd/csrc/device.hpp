@@ -73,6 +73,7 @@ struct alignas(64) DevState {
   u64 small_done;     // steps completed by the last LDS-resident run (k_small)
   u64 small_peak;     // largest live count inside that run
   u64 pad[2];
+  u64 sup[16];        // OR of the live entries' key words (x words, then z words), written at a sync
 };
 
 // Branching op: a rotation exp(-i theta/2 G) with G acting on qubits a (and

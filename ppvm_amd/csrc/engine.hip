@@ -135,7 +135,7 @@ struct ppvm_sum {
   u64* masks = nullptr;     // ballot masks, cap/64 + kMaxParts words
   u64* tp = nullptr;        // bit-sliced key planes, 2 n planes of tstride words (k_rot_count)
   u64* hsh = nullptr;       // pair-key hash per worklist record (k_rot_gather)
-  u64* bkt = nullptr;       // worklist records in bucket order (k_pair_scatter)
+  u64* bkt = nullptr;       // worklist records in bucket order (k_pair_scatter_tiled)
   u32* pmeta = nullptr;     // per-(bucket, slice) counts, then bucket bases and sizes
   u64 tstride = 0;
   u64* parts = nullptr;     // 6 x kMaxParts per-chunk counts / partial sums
@@ -453,11 +453,8 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
                        s->cap, lgm, it.seq + 1, hist);
     hipLaunchKernelGGL(k_pair_colscan, dim3((1u << lgm) / 64 + 1), dim3(kPartThreads), 0, s->stream, s->st, in,
                        s->cap, lgm, hist, bmeta);
-    // one of the two scatters does the work (by the gate's bucket count)
     hipLaunchKernelGGL(k_pair_scatter_tiled, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st,
                        in, s->cap, lgm, hist, s->bkt, bmeta);
-    hipLaunchKernelGGL(k_pair_scatter, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st, in,
-                       s->cap, lgm, hist, s->bkt, bmeta);
     hipLaunchKernelGGL(k_pair_lds<W>, dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, keys, wl, s->st, in,
                        it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(), s->part(1),
                        s->part(2));
@@ -814,10 +811,42 @@ int launch_fingerprint(ppvm_sum* s, u32 G) {
   return PPVM_OK;
 }
 
+// Support of the state after the last issued op into st->sup, so that the
+// read-back of the state carries it (no second round trip).
+int launch_support_dev(ppvm_sum* s) {
+  const u32 W = s->W;
+  const u32 G = (u32)grid_for(std::max<u64>(s->n_bound, 1), (u64)kBlock * 8, 1024);
+  int rc;
+  if ((rc = prof_begin(s, PH_OTHER))) return rc;
+  hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s->stream, s->st->sup, (u64)(2 * W), 0ull);
+  HIPCHK(hipGetLastError());
+  u64* none = nullptr;
+  switch (W) {
+    case 1: hipLaunchKernelGGL(k_support<1>, dim3(G), dim3(kBlock), 0, s->stream, s->keys[s->cur], s->coef[s->cur],
+                               s->st, (u32)(s->op_seq & 1), none); break;
+    case 2: hipLaunchKernelGGL(k_support<2>, dim3(G), dim3(kBlock), 0, s->stream, s->keys[s->cur], s->coef[s->cur],
+                               s->st, (u32)(s->op_seq & 1), none); break;
+    case 4: hipLaunchKernelGGL(k_support<4>, dim3(G), dim3(kBlock), 0, s->stream, s->keys[s->cur], s->coef[s->cur],
+                               s->st, (u32)(s->op_seq & 1), none); break;
+    default: hipLaunchKernelGGL(k_support<8>, dim3(G), dim3(kBlock), 0, s->stream, s->keys[s->cur],
+                                s->coef[s->cur], s->st, (u32)(s->op_seq & 1), none); break;
+  }
+  HIPCHK(hipGetLastError());
+  return prof_end(s);
+}
+
 int sync_check(ppvm_sum* s) {
   if (s->broken) return fail(PPVM_ESTATE, "handle is unusable after an earlier device error");
   if (s->fresh) return PPVM_OK;
   int rc;
+  // the support rides along with the state read-back when the arena is small
+  // enough for the pass to pay (an upper bound decides; it stays valid across
+  // compaction and growth, which move live entries without changing them)
+  bool sup_ok = s->n_bound <= kSupportRefreshMax;
+  if (sup_ok && (rc = launch_support_dev(s))) {
+    s->broken = true;
+    return rc;
+  }
   for (int round = 0; round < 64; ++round) {
     if ((rc = read_state(s))) {
       s->broken = true;
@@ -825,6 +854,7 @@ int sync_check(ppvm_sum* s) {
     }
     const u64 of = s->st_host->overflow;
     if (!of) break;
+    sup_ok = false;  // the state is rebuilt by the replay below
     const u64 seq = of - 1;
     const CountSlot& c = s->st_host->slot[seq & 1];
     const u64 live = c.n_entries - c.n_dead;
@@ -880,9 +910,14 @@ int sync_check(ppvm_sum* s) {
     }
     s->hs.regrows += 1;
   }
-  if (c2.n_entries <= kSupportRefreshMax && (rc = refresh_support(s, c2.n_entries))) {
-    s->broken = true;
-    return rc;
+  if (c2.n_entries <= kSupportRefreshMax) {
+    if (sup_ok) {
+      s->xsup.assign(s->st_host->sup, s->st_host->sup + s->W);
+      s->zsup.assign(s->st_host->sup + s->W, s->st_host->sup + 2 * s->W);
+    } else if ((rc = refresh_support(s, c2.n_entries))) {
+      s->broken = true;
+      return rc;
+    }
   }
   s->fresh = true;
   return PPVM_OK;

@@ -7,7 +7,7 @@
 //                  arena index, the child's sign and the truncation tests of
 //                  the key and its child) plus its pair-key hash, in arena
 //                  order (dead entries are flagged).
-//   k_pair_hist, k_pair_scatter, k_pair_lds : radix-partition the worklist
+//   k_pair_hist, k_pair_colscan, k_pair_scatter_tiled, k_pair_lds : radix-partition the worklist
 //                  on the hash of the pair key (key with the generator's pivot
 //                  bit cleared) and pair each bucket in an LDS hash table; the
 //                  second member of a pair to arrive computes both merged
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ k
 //    clearing unnecessary) with device-scope CAS; cheapest for small gates.
 //  * the partitioned path: the worklist is radix-partitioned on the top lg
 //    bits of h into 2^lg buckets (k_pair_hist counts per (slice, bucket),
-//    k_pair_colscan turns the counts into per-slice offsets, k_pair_scatter
+//    k_pair_colscan turns the counts into per-slice offsets, k_pair_scatter_tiled
 //    places (low 32 bits of h, record) bucket-contiguously) and k_pair_lds
 //    pairs each bucket in an LDS table, so global memory sees a few 8-byte
 //    words per record plus the merged pairs' two records.
@@ -626,16 +626,60 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_colscan(const DevState* _
   }
 }
 
+// ---- per-record scatter (more than 2^kTiledLgMax buckets) --------------------------
+// Slice s writes bkt[pos] = record | (low 32 bits of h) << 32, at its offset
+// in the record's bucket: bucket base (prefix of the bucket sizes) plus the
+// counts of the slices before s.  Block 0 publishes the bucket bases in
+// bmeta[0..B).  `off` is LDS for kMaxBuckets counters.
+__device__ __forceinline__ void pair_scatter_records(const u64* __restrict__ hsh, u64 n_wl, u32 lg,
+                                                     const u32* __restrict__ hist, u64* __restrict__ bkt,
+                                                     u32* __restrict__ bmeta, u32* off, u32* wsum) {
+  const u32 nb = 1u << lg;
+  const u32 s = blockIdx.x;
+  // bucket bases: exclusive scan of the sizes, kPer consecutive buckets per thread
+  constexpr u32 kPer = kMaxBuckets / kPartThreads;
+  const u32 b0 = threadIdx.x * kPer;
+  const u32* row = hist + (u64)s * nb;
+  u32 loc[kPer], pre[kPer], sum = 0;
+#pragma unroll
+  for (u32 q = 0; q < kPer; ++q) {
+    const bool on = b0 + q < nb;
+    loc[q] = on ? bmeta[kMaxBuckets + b0 + q] : 0u;
+    pre[q] = on ? row[b0 + q] : 0u;
+    sum += loc[q];
+  }
+  u32 base = block_excl_scan_u32(sum, wsum);
+#pragma unroll
+  for (u32 q = 0; q < kPer; ++q) {
+    if (b0 + q < nb) {
+      if (s == 0) bmeta[b0 + q] = base;
+      off[b0 + q] = base + pre[q];
+    }
+    base += loc[q];
+  }
+  __syncthreads();
+  const u64 lo = n_wl * s / kPairSlices, hi = n_wl * (s + 1) / kPairSlices;
+#pragma unroll 4
+  for (u64 r = lo + threadIdx.x; r < hi; r += kPartThreads) {
+    const u64 v = hsh[r];
+    if (v == kDeadHash) continue;
+    const u32 pos = atomicAdd(&off[bucket_of(v, lg)], 1u);
+    bkt[pos] = r | ((u64)(u32)v << 32);
+  }
+}
+
 // ---- k_pair_scatter_tiled ---------------------------------------------------------
 // The scatter for up to 2^kTiledLgMax buckets: slice s takes its records in
 // tiles of kScatterTile, sorts each tile by bucket in LDS (local counts, scan,
 // placement) and then writes the tile in that order, so consecutive threads
 // store consecutive addresses of one bucket's run instead of one 8-byte store
-// per bucket line (k_pair_scatter, used for more buckets, does the latter).
+// per bucket line (the per-record scatter, used for more buckets, does the
+// latter).
 // Results are the same multiset per bucket; the order inside a bucket does
 // not matter to k_pair_lds.
 constexpr u32 kTiledLgMax = 12;
 constexpr u32 kScatterTile = 8192;
+static_assert(kScatterTile * 2 >= kMaxBuckets, "the tile buffer holds the per-record scatter's counters");
 constexpr u32 kTilePer = kScatterTile / kPartThreads;
 constexpr u32 kTiledBuckets = 1u << kTiledLgMax;
 constexpr u32 kTiledPer = kTiledBuckets / kPartThreads;
@@ -656,7 +700,13 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_scatter_tiled(const u64* 
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
   const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
-  if (lg > kTiledLgMax) return;  // k_pair_scatter places this gate's records
+  if (lg > kTiledLgMax) {
+    // too many buckets for the LDS tile: one 4-byte counter per bucket in
+    // the tile buffer (kScatterTile u64 = kMaxBuckets u32) and a scattered
+    // store per record
+    pair_scatter_records(hsh, n_wl, lg, hist, bkt, bmeta, reinterpret_cast<u32*>(buf), wsum);
+    return;
+  }
   const u32 s = blockIdx.x, tid = threadIdx.x;
   // bucket bases (exclusive scan of the sizes; block 0 publishes them) and this
   // slice's offset inside each bucket
@@ -737,56 +787,6 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_scatter_tiled(const u64* 
     __syncthreads();
     for (u32 b = tid; b < nb; b += kPartThreads) off[b] += tcnt[b];
     __syncthreads();
-  }
-}
-
-// ---- k_pair_scatter -------------------------------------------------------------
-// Slice s writes bkt[pos] = record | (low 32 bits of h) << 32, at its offset
-// in the record's bucket: bucket base (prefix of the bucket sizes) plus the
-// counts of the slices before s.  Block 0 publishes the bucket bases in
-// bmeta[0..B).
-__global__ __launch_bounds__(kPartThreads) void k_pair_scatter(const u64* __restrict__ hsh,
-                                                              const DevState* __restrict__ st, u32 in_slot,
-                                                              u64 cap, u32 lg_max, const u32* __restrict__ hist,
-                                                              u64* __restrict__ bkt, u32* __restrict__ bmeta) {
-  __shared__ u32 off[kMaxBuckets];
-  __shared__ u32 wsum[kPartThreads / 64];
-  if (st->overflow) return;
-  const u64 n_entries = st->slot[in_slot].n_entries;
-  const u64 n_wl = st->n_wl;
-  if (n_entries + n_wl > cap) return;
-  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
-  if (lg <= kTiledLgMax) return;  // k_pair_scatter_tiled places this gate's records
-  const u32 s = blockIdx.x;
-  // bucket bases: exclusive scan of the sizes, kPer consecutive buckets per thread
-  constexpr u32 kPer = kMaxBuckets / kPartThreads;
-  const u32 b0 = threadIdx.x * kPer;
-  const u32* row = hist + (u64)s * nb;
-  u32 loc[kPer], pre[kPer], sum = 0;
-#pragma unroll
-  for (u32 q = 0; q < kPer; ++q) {
-    const bool on = b0 + q < nb;
-    loc[q] = on ? bmeta[kMaxBuckets + b0 + q] : 0u;
-    pre[q] = on ? row[b0 + q] : 0u;
-    sum += loc[q];
-  }
-  u32 base = block_excl_scan_u32(sum, wsum);
-#pragma unroll
-  for (u32 q = 0; q < kPer; ++q) {
-    if (b0 + q < nb) {
-      if (s == 0) bmeta[b0 + q] = base;
-      off[b0 + q] = base + pre[q];
-    }
-    base += loc[q];
-  }
-  __syncthreads();
-  const u64 lo = n_wl * s / kPairSlices, hi = n_wl * (s + 1) / kPairSlices;
-#pragma unroll 4
-  for (u64 r = lo + threadIdx.x; r < hi; r += kPartThreads) {
-    const u64 v = hsh[r];
-    if (v == kDeadHash) continue;
-    const u32 pos = atomicAdd(&off[bucket_of(v, lg)], 1u);
-    bkt[pos] = r | ((u64)(u32)v << 32);
   }
 }
 
@@ -1134,12 +1134,13 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ keys, d
 
 // ---- k_support: union of the live terms' x and z words --------------------------
 // Per block, the OR of every live entry's 2W key words, written to
-// out[block * 2W + p]; the host ORs the blocks.  Lets the host judge exactly
-// which rotations can act on some term (see k_skip_gates).
+// out[block * 2W + p] for the host to OR, or (out null) OR-ed into st->sup,
+// which a k_fill_u64 cleared before.  Lets the host judge exactly which
+// rotations can act on some term (see k_skip_gates).
 template <int W>
 __global__ __launch_bounds__(kBlock) void k_support(const u64* __restrict__ keys,
                                                     const double* __restrict__ coef,
-                                                    const DevState* __restrict__ st, u32 in_slot,
+                                                    DevState* __restrict__ st, u32 in_slot,
                                                     u64* __restrict__ out) {
   __shared__ u64 lds[2 * W][kWaves];
   const u64 n = st->slot[in_slot].n_entries;
@@ -1162,7 +1163,11 @@ __global__ __launch_bounds__(kBlock) void k_support(const u64* __restrict__ keys
   if (threadIdx.x < 2 * W) {
     u64 v = 0;
     for (int w = 0; w < kWaves; ++w) v |= lds[threadIdx.x][w];
-    out[(u64)blockIdx.x * (2 * W) + threadIdx.x] = v;
+    if (out) {
+      out[(u64)blockIdx.x * (2 * W) + threadIdx.x] = v;
+    } else if (v) {
+      atomicOr(reinterpret_cast<unsigned long long*>(&st->sup[threadIdx.x]), (unsigned long long)v);
+    }
   }
 }
 
