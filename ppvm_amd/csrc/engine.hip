@@ -427,8 +427,8 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
   // chunked kernels loop over their chunks, so the grids stay near one
   // resident wave of blocks however large the arena is
   const u64 ne = grid_entries(s);
-  const u32 G = std::min<u32>(chunk_blocks(ne, kArenaChunk), kArenaGrid);
-  const u32 Gl = std::min<u32>(chunk_blocks(ne, kListChunk), kListGrid);
+  const u32 G = std::min<u32>(chunk_blocks(ne, chunk_target(ne, kArenaChunk)), kArenaGrid);
+  const u32 Gl = std::min<u32>(chunk_blocks(ne, chunk_target(ne, kListChunk)), kListGrid);
   WMRec* wl = s->wl;
   u64* keys = s->keys[s->cur];
   double* coef = s->coef[s->cur];

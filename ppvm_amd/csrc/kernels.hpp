@@ -42,8 +42,22 @@ namespace ppvm {
 // chunk's offset from the counts of the chunks before it and ranks the items
 // inside the chunk with a scan of the mask popcounts in LDS.
 constexpr int kChunkMaxWords = 1024;
-constexpr u32 kArenaChunk = 8192;   // target items per chunk, arena passes
-constexpr u32 kListChunk = 2048;    // target items per chunk, worklist passes
+constexpr u32 kArenaChunk = 8192;   // largest target items per chunk, arena passes
+constexpr u32 kListChunk = 2048;    // largest target items per chunk, worklist passes
+constexpr u32 kChunksAimed = 1024;  // chunks aimed at for a pass over n items
+
+// Target chunk size of the rotation passes for n items: about kChunksAimed
+// chunks of at least 256 items, at most `max_target` items each, so that a
+// gate on a small arena still spreads over hundreds of workgroups.  The
+// passes loop over their chunks, so a launch sized from an upper bound of n
+// covers any actual n (blocks beyond the chunk count exit at once).
+__host__ __device__ __forceinline__ u32 chunk_target(u64 n, u32 max_target) {
+  u64 t = (n + kChunksAimed - 1) / kChunksAimed;
+  t = (t + 255) & ~255ull;
+  if (t < 256) t = 256;
+  if (t > max_target) t = max_target;
+  return (u32)t;
+}
 
 // Effective chunk count for n items with a target chunk size `target`
 // (computed on the device from the actual n, so the launch may carry extra
@@ -230,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_count(const u64* __restrict__ ke
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const u64 n = st->slot[in_slot].n_entries;
   const u64 tvw = st->tvw;
-  const u32 G = chunk_count(n, kArenaChunk);
+  const u32 G = chunk_count(n, chunk_target(n, kArenaChunk));
   const u64* pax = tp + (u64)g.a * tstride;
   const u64* paz = tp + (u64)(nq + g.a) * tstride;
   const u64* pbx = tp + (u64)g.b * tstride;
@@ -355,7 +369,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ k
   __shared__ u32 lt[kBlock];
   if (st->overflow) return;
   const u64 n = st->slot[in_slot].n_entries;
-  const u32 G = chunk_count(n, kArenaChunk);
+  const u32 G = chunk_count(n, chunk_target(n, kArenaChunk));
   if (blockIdx.x == 0) {
     const u64 tot = chunk_offset(cnt, G, lds);
     if (threadIdx.x == 0) {
@@ -1019,7 +1033,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_count(double* __restrict__ coef
   const u64 n_entries = st->slot[in_slot].n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
-  const u32 G = chunk_count(n_wl, kListChunk);
+  const u32 G = chunk_count(n_wl, chunk_target(n_wl, kListChunk));
   const u32 lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   for (u32 ch = blockIdx.x; ch < G; ch += gridDim.x) {
     u64 lo, hi;
@@ -1087,7 +1101,7 @@ __global__ __launch_bounds__(kBlock) void k_emit_write(u64* __restrict__ keys, d
   const u64 n_entries = in.n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
-  const u32 G = chunk_count(n_wl, kListChunk);
+  const u32 G = chunk_count(n_wl, chunk_target(n_wl, kListChunk));
   if (blockIdx.x == 0) {
     u64 sums[4];
     block_sums4(cnt, G, ekills, G, pkills, Gp, ppairs, Gp, lds4, sums);
