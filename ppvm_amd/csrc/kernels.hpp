@@ -420,15 +420,15 @@ __global__ __launch_bounds__(kBlock) void k_rot_gather(const u64* __restrict__ k
 //    pairs each bucket in an LDS table, so global memory sees a few 8-byte
 //    words per record plus the merged pairs' two records.
 // Bits of h on the partitioned path: bucket = top lg bits (lg <= 14); LDS
-// slot = bits 0..12; fingerprint = bits 13..24; sub-bin = bits 26..31.
+// slot = bits 0..13; fingerprint = bits 14..25; sub-bin = bits 26..31.
 constexpr u32 kPairSlices = 256;     // worklist slices of the partition kernels
 constexpr u32 kPartThreads = 1024;   // threads per block of the partition kernels
 constexpr u32 kMaxBucketsLg = 14;
 constexpr u32 kMaxBuckets = 1u << kMaxBucketsLg;
-constexpr u32 kBucketTarget = 2048;  // records per bucket aimed at
+constexpr u32 kBucketTarget = 6144;  // records per bucket aimed at (buckets average 3072-6144)
 constexpr u32 kPairThreads = 512;    // threads per block of k_pair_lds
-constexpr u32 kLdsSlotsLg = 13;
-constexpr u32 kLdsSlots = 1u << kLdsSlotsLg;  // 32 KB of u32 slots
+constexpr u32 kLdsSlotsLg = 14;
+constexpr u32 kLdsSlots = 1u << kLdsSlotsLg;  // 64 KB of u32 slots
 constexpr u32 kPassMax = kLdsSlots / 2;       // records per LDS pass (load factor <= 1/2)
 constexpr u32 kPerThread = kPassMax / kPairThreads;
 constexpr u32 kSubBins = 64;

@@ -3,8 +3,8 @@
 With PPVM_PARTITION_MAX_LG=0 the radix partition keeps the whole worklist in
 one bucket, which takes the branches that ordinary workloads reach only past
 tens of millions of anticommuting terms: pairing in several LDS passes over
-sub-bins (bucket > 4096 records), the global table for sub-bins that are still
-too large (bucket > ~262k records), and the global table for a whole bucket
+sub-bins (bucket > 8192 records), the global table for sub-bins that are still
+too large (bucket > ~524k records), and the global table for a whole bucket
 whose positions exceed 20 bits (> 1,048,575 records).  Every term anticommutes
 with the first gate, Z on qubit 0, and half of them have their partner (Z on
 qubit 0 flipped) in the sum.  Bar: identical keys, order and coefficient bits.
@@ -19,6 +19,8 @@ from tests.helpers import assert_same_terms, run_segments
 pytestmark = pytest.mark.gpu
 
 N = 70  # two 64-bit words per plane
+PASS_MAX = 8192  # records one LDS pass of the pairing kernel holds (half its table)
+SUB_BINS = 64
 
 
 def _words(rng, m):
@@ -41,7 +43,7 @@ def _words(rng, m):
     return allk[:, :2].copy(), allk[:, 2:].copy(), c
 
 
-@pytest.mark.parametrize("m", [12_000, 200_000, 720_000])
+@pytest.mark.parametrize("m", [12_000, 200_000, 450_000, 720_000])
 def test_single_bucket_branches(m, monkeypatch):
     from oracle.pyoracle import OracleSum
     from ppvm_amd import PauliSum
@@ -66,9 +68,9 @@ def test_single_bucket_branches(m, monkeypatch):
     print(f"m={m}: {total} records anticommute with the first gate", flush=True)
     run_segments(g, r, prog, list(range(len(prog))), ctx=f"single bucket m={m}")
     fb = g._h.counters()["pair_fallback"]
-    if total <= 64 * 4096 // 2:
+    if total <= SUB_BINS * PASS_MAX // 2:
         assert fb == 0, f"{fb} records took the global table"
     if total > (1 << 20):
         assert fb >= total, f"only {fb} of {total} records took the global table"
-    elif total > 64 * 4096:
+    elif total > SUB_BINS * PASS_MAX:
         assert fb > 0
