@@ -360,7 +360,7 @@ u32 chunk_blocks(u64 bound, u32 target) {
 constexpr int kFixedGrid = 2048;  // grid-stride kernels with per-block partials
 constexpr u32 kArenaGrid = 4096;  // blocks of the chunked arena passes (count, gather)
 constexpr u32 kListGrid = 2048;   // blocks of the chunked worklist passes (emit)
-constexpr u32 kPairGrid = 512;    // blocks of k_pair_lds (2 resident per CU; each loops over buckets)
+constexpr u32 kPairGrid = 256;    // blocks of k_pair_lds (one resident per CU; each loops over buckets)
 // Arena size (at the last synchronisation) from which rotations pair through
 // the radix partition instead of the single global-table kernel; the
 // environment variable PPVM_PARTITION_MIN_ENTRIES overrides it per handle

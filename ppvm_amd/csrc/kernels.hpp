@@ -426,7 +426,7 @@ constexpr u32 kPartThreads = 1024;   // threads per block of the partition kerne
 constexpr u32 kMaxBucketsLg = 14;
 constexpr u32 kMaxBuckets = 1u << kMaxBucketsLg;
 constexpr u32 kBucketTarget = 6144;  // records per bucket aimed at (buckets average 3072-6144)
-constexpr u32 kPairThreads = 512;    // threads per block of k_pair_lds
+constexpr u32 kPairThreads = 1024;   // threads per block of k_pair_lds
 constexpr u32 kLdsSlotsLg = 14;
 constexpr u32 kLdsSlots = 1u << kLdsSlotsLg;  // 64 KB of u32 slots
 constexpr u32 kPassMax = kLdsSlots / 2;       // records per LDS pass (load factor <= 1/2)
@@ -857,6 +857,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap, const u64* _
                                                            u64* __restrict__ part_pairs) {
   __shared__ __align__(16) u32 tab[kLdsSlots];
   __shared__ u32 cand[kPassMax];
+  __shared__ u32 lrec[kPassMax];  // record index of each bucket position (single pass)
   __shared__ u32 sub[kSubBins];
   __shared__ u32 plo[kSubBins], phi[kSubBins], pfb[kSubBins];
   __shared__ u32 npass, ncand;
@@ -882,6 +883,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap, const u64* _
       for (u32 q = 0; q < kPerThread; ++q) {
         const u32 e = tid + q * kPairThreads;
         v[q] = e < T ? bkt[base + e] : 0ull;
+        if (e < T) lrec[e] = (u32)v[q];
       }
       __syncthreads();
 #pragma unroll
@@ -915,7 +917,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap, const u64* _
       bool retry_any = false;
       for (u32 c = tid; c < nc; c += kPairThreads) {
         const u32 ce = cand[c] & 0xffffu, cj = cand[c] >> 16;
-        const u32 i = (u32)bkt[base + ce], j = (u32)bkt[base + cj];
+        const u32 i = lrec[ce], j = lrec[cj];
         if (pair_try_merge<W>(keys, wm, i, j, g, t, kills)) {
           ++pairs;
         } else {
