@@ -162,6 +162,7 @@ struct ppvm_sum {
   u64 n_synced = 0; // arena entries at the last host synchronisation (path choice)
   u64 partition_min = 0;  // pair through the radix partition from this many entries on
   u32 bucket_lg_max = 0;  // at most 2^bucket_lg_max buckets in the partition
+  bool pair_sort = false; // sort-based pairing inside the partition's buckets
   ppvm_stats hs{};
   // LDS-resident runs of small sums: while small_on, the planner collects its
   // steps in small_list instead of issuing them, and small_flush runs them in
@@ -383,6 +384,15 @@ u32 bucket_lg_limit() {
   return (end && *end == '\0' && v < kMaxBucketsLg) ? (u32)v : kMaxBucketsLg;
 }
 
+// Pairing inside a bucket of the partition: the LDS hash table (default) or
+// the sort-based variant (PPVM_PAIR_METHOD=sort: counting sort on the low
+// hash bits, then a merge over runs of equal bits).  Both give identical
+// results; the default was chosen by timing both on the lattice workload.
+bool pair_sort_env() {
+  const char* e = std::getenv("PPVM_PAIR_METHOD");
+  return e && std::strcmp(e, "sort") == 0;
+}
+
 // Sums whose synchronised arena holds at most this many entries run their
 // pending steps in LDS (k_small): three quarters of the LDS capacity by
 // default, so a run has room to grow (a run that outgrows LDS hands over to
@@ -455,9 +465,14 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
                        s->cap, lgm, hist, bmeta);
     hipLaunchKernelGGL(k_pair_scatter_tiled, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st,
                        in, s->cap, lgm, hist, s->bkt, bmeta);
-    hipLaunchKernelGGL(k_pair_lds<W>, dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, keys, wl, s->st, in,
-                       it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(), s->part(1),
-                       s->part(2));
+    if (s->pair_sort)
+      hipLaunchKernelGGL((k_pair_lds<W, true>), dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, keys, wl,
+                         s->st, in, it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(),
+                         s->part(1), s->part(2));
+    else
+      hipLaunchKernelGGL((k_pair_lds<W, false>), dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, keys, wl,
+                         s->st, in, it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(),
+                         s->part(1), s->part(2));
   } else {
     Gp = (u32)grid_for(ne, kBlock * 2, kFixedGrid);
     hipLaunchKernelGGL(k_rot_pair<W>, dim3(Gp), dim3(kBlock), 0, s->stream, s->cap, keys, wl, s->st, in,
@@ -1561,6 +1576,7 @@ int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint,
   s->trunc_active = tau > 0.0 || maxw != ~0ull;
   s->partition_min = partition_min_entries();
   s->bucket_lg_max = bucket_lg_limit();
+  s->pair_sort = pair_sort_env();
   s->small_max = small_max_entries(Wp);
   s->small_off = s->small_max == 0;
   s->small_threads = small_threads_env();
@@ -1703,6 +1719,7 @@ int ppvm_clone(ppvm_handle h, ppvm_handle* out) {
   c->zsup = h->zsup;
   c->sup_stale = h->sup_stale;
   c->bucket_lg_max = h->bucket_lg_max;
+  c->pair_sort = h->pair_sort;
   c->clean = h->clean;
   c->preserve_host = h->preserve_host;
   if ((rc = upload_preserve(c))) {

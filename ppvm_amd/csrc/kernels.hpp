@@ -432,6 +432,10 @@ constexpr u32 kLdsSlots = 1u << kLdsSlotsLg;  // 64 KB of u32 slots
 constexpr u32 kPassMax = kLdsSlots / 2;       // records per LDS pass (load factor <= 1/2)
 constexpr u32 kPerThread = kPassMax / kPairThreads;
 constexpr u32 kSubBins = 64;
+constexpr u32 kSortBins = 8192;               // digit counters of the sort-based pairing (hash bits 0..12)
+constexpr u32 kSortPer = kSortBins / kPairThreads;
+static_assert(kSortBins <= kPassMax && kPassMax * 8 <= kLdsSlots * 4,
+              "the sort-based pairing keeps its counters in cand and its sorted entries in tab");
 constexpr u32 kPosLimit = (1u << 20) - 1;     // bucket positions held by a slot (20 bits)
 
 __device__ __forceinline__ u32 pair_bucket_lg(u64 n_wl, u32 lg_max) {
@@ -845,7 +849,7 @@ __device__ __forceinline__ u64 lds_probe_sync(const u64* __restrict__ keys, WMRe
   }
 }
 
-template <int W>
+template <int W, bool kSort>
 __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap, const u64* __restrict__ keys,
                                                            WMRec* __restrict__ wm, DevState* st,
                                                            u32 in_slot, RotSpec g, u32 lg_max,
@@ -874,6 +878,78 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap, const u64* _
   for (u32 b = blockIdx.x; b < nb; b += gridDim.x) {
     const u32 base = bmeta[b], T = bmeta[kMaxBuckets + b];
     if (T == 0u) continue;
+    if (kSort && T <= kPassMax) {
+      // ---- single pass, sort-based: counting sort of the bucket on hash bits
+      // 0..12 in LDS (cand holds the 8192 digit counters, tab the sorted
+      // entries), then each run of equal digits is searched for equal 32-bit
+      // hashes, confirmed on the keys and merged (a key has at most one
+      // partner, so a record merges at most once).
+      static_assert(kPairThreads == kPartThreads, "block_excl_scan_u32 scans kPartThreads threads");
+      u32* cnt = cand;
+      u64* srt = reinterpret_cast<u64*>(tab);
+      for (u32 e = tid; e < kSortBins; e += kPairThreads) cnt[e] = 0u;
+      __syncthreads();
+      u64 v[kPerThread];
+      u32 rk[kPerThread];
+#pragma unroll
+      for (u32 q = 0; q < kPerThread; ++q) {
+        const u32 e = tid + q * kPairThreads;
+        v[q] = e < T ? bkt[base + e] : 0ull;
+        rk[q] = e < T ? atomicAdd(&cnt[(u32)(v[q] >> 32) & (kSortBins - 1u)], 1u) : 0u;
+      }
+      __syncthreads();
+      {  // digit starts: exclusive scan of the counters, kSortPer per thread
+        const u32 b0 = tid * kSortPer;
+        u32 c[kSortPer], sum = 0;
+#pragma unroll
+        for (u32 q = 0; q < kSortPer; ++q) {
+          c[q] = cnt[b0 + q];
+          sum += c[q];
+        }
+        u32 run = block_excl_scan_u32(sum, reinterpret_cast<u32*>(lds_sum));
+#pragma unroll
+        for (u32 q = 0; q < kSortPer; ++q) {
+          cnt[b0 + q] = run;
+          run += c[q];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (u32 q = 0; q < kPerThread; ++q) {
+        const u32 e = tid + q * kPairThreads;
+        if (e < T) srt[cnt[(u32)(v[q] >> 32) & (kSortBins - 1u)] + rk[q]] = v[q];
+      }
+      __syncthreads();
+      for (u32 p = tid; p < T; p += kPairThreads) {
+        const u32 d = (u32)(srt[p] >> 32) & (kSortBins - 1u);
+        if (p > 0 && ((u32)(srt[p - 1] >> 32) & (kSortBins - 1u)) == d) continue;  // not a run start
+        u32 end = p + 1;
+        while (end < T && ((u32)(srt[end] >> 32) & (kSortBins - 1u)) == d) ++end;
+        if (end - p > 32u) {  // a run beyond the merged-member mask: pair it through the global table
+          for (u32 a = p; a < end; ++a) {
+            const u32 i = (u32)srt[a];
+            pairs += pair_global<W>(keys, wm, i, hsh[i], table, tmask, tgen, g, t, kills);
+            ++fallback;
+          }
+          continue;
+        }
+        u32 done = 0;  // run members already merged (runs are short: the bucket has ~T/8192 per digit)
+        for (u32 a = p; a + 1 < end; ++a) {
+          if ((done >> (a - p)) & 1u) continue;
+          for (u32 b = a + 1; b < end; ++b) {
+            if ((done >> (b - p)) & 1u) continue;
+            if ((u32)(srt[a] >> 32) != (u32)(srt[b] >> 32)) continue;
+            if (pair_try_merge<W>(keys, wm, (u32)srt[a], (u32)srt[b], g, t, kills)) {
+              ++pairs;
+              done |= 1u << (b - p);
+              break;
+            }
+          }
+        }
+      }
+      __syncthreads();  // counters and sorted entries are rewritten for the next bucket
+      continue;
+    }
     if (T <= kPassMax) {
       // ---- single pass
       for (u32 e = tid; e < kLdsSlots / 4; e += kPairThreads) t4[e] = make_uint4(0u, 0u, 0u, 0u);

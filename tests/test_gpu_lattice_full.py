@@ -82,7 +82,16 @@ def test_device_fingerprint_matches_definition(wname):
 
 
 @pytest.mark.gpu
-def test_lattice_full_run_every_step():
+@pytest.mark.parametrize("method", ["hash", "sort"])
+def test_lattice_full_run_every_step(method, monkeypatch):
+    """Default settings, with the LDS hash table (the default) or the
+    sort-based pairing inside the partition's buckets."""
+    for k in ("PPVM_PARTITION_MIN_ENTRIES", "PPVM_SMALL_MAX_ENTRIES", "PPVM_PARTITION_MAX_LG"):
+        monkeypatch.delenv(k, raising=False)
+    if method == "sort":
+        monkeypatch.setenv("PPVM_PAIR_METHOD", "sort")
+    else:
+        monkeypatch.delenv("PPVM_PAIR_METHOD", raising=False)
     w = WL.lattice_tfim()
     rows = _golden()["steps"]
     g = gpu_for(w)  # default settings: grows from a small arena

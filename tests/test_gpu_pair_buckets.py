@@ -43,13 +43,21 @@ def _words(rng, m):
     return allk[:, :2].copy(), allk[:, 2:].copy(), c
 
 
-@pytest.mark.parametrize("m", [12_000, 200_000, 450_000, 720_000])
-def test_single_bucket_branches(m, monkeypatch):
+@pytest.mark.parametrize("method", ["hash", "sort"])
+@pytest.mark.parametrize("m", [4_000, 12_000, 200_000, 450_000, 720_000])
+def test_single_bucket_branches(m, method, monkeypatch):
+    """m = 4,000 fits one pass (6,000 records), which the sort-based pairing
+    handles by its counting sort; the larger buckets take the shared
+    multi-pass and global-table branches."""
     from oracle.pyoracle import OracleSum
     from ppvm_amd import PauliSum
     monkeypatch.setenv("PPVM_PARTITION_MIN_ENTRIES", "0")
     monkeypatch.setenv("PPVM_SMALL_MAX_ENTRIES", "0")
     monkeypatch.setenv("PPVM_PARTITION_MAX_LG", "0")
+    if method == "sort":
+        monkeypatch.setenv("PPVM_PAIR_METHOD", "sort")
+    else:
+        monkeypatch.delenv("PPVM_PAIR_METHOD", raising=False)
     rng = np.random.default_rng(m)
     x, z, c = _words(rng, m)
     total = len(c)

@@ -17,21 +17,25 @@ from tests.helpers import (assert_same_terms, gpu_for, oracle_for, random_progra
 pytestmark = pytest.mark.gpu
 
 
-@pytest.fixture(autouse=True, params=["global", "partition", "small"])
+@pytest.fixture(autouse=True, params=["global", "partition", "partition_sort", "small"])
 def pairing_path(request, monkeypatch):
     """Run every test on each execution path: the arena pipeline with the
     single global-table pairing kernel, the arena pipeline with the radix
-    partition + LDS pairing kernels (forced at every size), and the default
-    settings, under which a small sum runs its steps in LDS in one workgroup
-    (k_small) and hands over to the arena pipeline when it outgrows LDS.
-    Handles read the choice when they are created."""
+    partition + LDS pairing kernels (forced at every size), the same with the
+    sort-based pairing inside the buckets, and the default settings, under
+    which a small sum runs its steps in LDS in one workgroup (k_small) and
+    hands over to the arena pipeline when it outgrows LDS.  Handles read the
+    choice when they are created."""
+    monkeypatch.delenv("PPVM_PAIR_METHOD", raising=False)
     if request.param == "small":
         monkeypatch.delenv("PPVM_PARTITION_MIN_ENTRIES", raising=False)
         monkeypatch.delenv("PPVM_SMALL_MAX_ENTRIES", raising=False)
     else:
         monkeypatch.setenv("PPVM_SMALL_MAX_ENTRIES", "0")
         monkeypatch.setenv("PPVM_PARTITION_MIN_ENTRIES",
-                           "0" if request.param == "partition" else str(2**63))
+                           str(2**63) if request.param == "global" else "0")
+        if request.param == "partition_sort":
+            monkeypatch.setenv("PPVM_PAIR_METHOD", "sort")
     return request.param
 
 

@@ -8,13 +8,13 @@ fallback bound on the ensemble std is calibrated on that split, and the other
 1024 points are queried (known pairs by lookup, new pairs by the ensemble
 under the bound, the propagator above it).  The plain sweep of the same query
 points runs live in the same test; its values judge every error and must
-reproduce the shipped grid exactly.
+reproduce the shipped grid (same term sets; overlaps within 1e-10).
 
 Measured on one MI355X: the calibrated bound sends every new pair to the
 propagator (the ensemble agrees on points where the propagator's output jumps
 by more than 1e-3 between neighbouring h values), so at |err| <= 1e-3 the
 query phase gains only what the known pairs save (1.1x).  At the fixed std
-bound of 1e-2 the gain is 20x, with 88% of the points within 1e-3 and the
+bound of 1e-2 the gain is 12x, with 88% of the points within 1e-3 and the
 rest within 1e-2.
 """
 
