@@ -87,3 +87,32 @@ def test_small_every_word_count(n):
     run_segments(g, r, prog, list(range(7, len(prog), 40)), ctx=f"n={n}")
     assert_same_terms(g._export(), r.export(), True, f"n={n} end")
     assert g._h.counters()["small_runs"] >= 1
+
+
+def test_hand_over_launches_no_rotation_that_acts_on_no_term(monkeypatch):
+    """The steps an LDS run leaves when it outgrows LDS were planned with the
+    support widened over the whole collection; the hand-over re-decides them
+    from the exact support of the state the run left.  A lattice prefix that
+    starts in LDS therefore launches no more rotation pipelines than the same
+    run on the arena pipeline alone (where the support is exact at every
+    synchronisation), with the same results."""
+    w = WL.truncate_steps(WL.lattice_tfim(), 8)
+
+    def launched(small):
+        if small:
+            monkeypatch.delenv("PPVM_SMALL_MAX_ENTRIES", raising=False)
+        else:
+            monkeypatch.setenv("PPVM_SMALL_MAX_ENTRIES", "0")
+        g = gpu_for(w)
+        g._h.profile(True)
+        g.apply(w.program)
+        g.synchronize()
+        _, n = g._h.profile_read()
+        return int(n[0]), g._h.counters(), g._h.fingerprint()
+
+    a, ca, fa = launched(True)
+    b, cb, fb = launched(False)
+    print(f"rotation pipelines launched: {a} with the LDS start, {b} without", flush=True)
+    assert ca["small_runs"] >= 1 and cb["small_runs"] == 0
+    assert fa == fb and ca["term_updates"] == cb["term_updates"]
+    assert 0 < a <= b
