@@ -163,6 +163,7 @@ struct ppvm_sum {
   u64 partition_min = 0;  // pair through the radix partition from this many entries on
   u32 bucket_lg_max = 0;  // at most 2^bucket_lg_max buckets in the partition
   bool pair_sort = false; // sort-based pairing inside the partition's buckets
+  u32 bucket_target = 0;  // records per bucket aimed at
   ppvm_stats hs{};
   // LDS-resident runs of small sums: while small_on, the planner collects its
   // steps in small_list instead of issuing them, and small_flush runs them in
@@ -384,6 +385,17 @@ u32 bucket_lg_limit() {
   return (end && *end == '\0' && v < kMaxBucketsLg) ? (u32)v : kMaxBucketsLg;
 }
 
+// Records per bucket the partition aims at; PPVM_PARTITION_BUCKET_RECORDS
+// lowers it (more, smaller buckets: the per-record scatter and small LDS
+// passes at test sizes).
+u32 bucket_target_env() {
+  const char* e = std::getenv("PPVM_PARTITION_BUCKET_RECORDS");
+  if (!e || !*e) return kBucketTarget;
+  char* end = nullptr;
+  const unsigned long v = std::strtoul(e, &end, 10);
+  return (end && *end == '\0' && v >= 1 && v <= kBucketTarget) ? (u32)v : kBucketTarget;
+}
+
 // Pairing inside a bucket of the partition: the LDS hash table (default) or
 // the sort-based variant (PPVM_PAIR_METHOD=sort: counting sort on the low
 // hash bits, then a merge over runs of equal bits).  Both give identical
@@ -459,19 +471,20 @@ int launch_rot(ppvm_sum* s, const Issued& it) {
     u32* hist = s->pmeta;
     u32* bmeta = s->pmeta + (size_t)kMaxBuckets * kPairSlices;
     const u32 lgm = s->bucket_lg_max;
+    const PartSpec ps{lgm, s->bucket_target};
     hipLaunchKernelGGL(k_pair_hist, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st, in,
-                       s->cap, lgm, it.seq + 1, hist);
+                       s->cap, ps, it.seq + 1, hist);
     hipLaunchKernelGGL(k_pair_colscan, dim3((1u << lgm) / 64 + 1), dim3(kPartThreads), 0, s->stream, s->st, in,
-                       s->cap, lgm, hist, bmeta);
+                       s->cap, ps, hist, bmeta);
     hipLaunchKernelGGL(k_pair_scatter_tiled, dim3(kPairSlices), dim3(kPartThreads), 0, s->stream, s->hsh, s->st,
-                       in, s->cap, lgm, hist, s->bkt, bmeta);
+                       in, s->cap, ps, hist, s->bkt, bmeta);
     if (s->pair_sort)
       hipLaunchKernelGGL((k_pair_lds<W, true>), dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, keys, wl,
-                         s->st, in, it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(),
+                         s->st, in, it.rot, ps, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(),
                          s->part(1), s->part(2));
     else
       hipLaunchKernelGGL((k_pair_lds<W, false>), dim3(Gp), dim3(kPairThreads), 0, s->stream, s->cap, keys, wl,
-                         s->st, in, it.rot, lgm, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(),
+                         s->st, in, it.rot, ps, s->hsh, s->bkt, bmeta, s->table, s->tcap, tg, s->trunc_spec(),
                          s->part(1), s->part(2));
   } else {
     Gp = (u32)grid_for(ne, kBlock * 2, kFixedGrid);
@@ -1577,6 +1590,7 @@ int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint,
   s->partition_min = partition_min_entries();
   s->bucket_lg_max = bucket_lg_limit();
   s->pair_sort = pair_sort_env();
+  s->bucket_target = bucket_target_env();
   s->small_max = small_max_entries(Wp);
   s->small_off = s->small_max == 0;
   s->small_threads = small_threads_env();
@@ -1720,6 +1734,7 @@ int ppvm_clone(ppvm_handle h, ppvm_handle* out) {
   c->sup_stale = h->sup_stale;
   c->bucket_lg_max = h->bucket_lg_max;
   c->pair_sort = h->pair_sort;
+  c->bucket_target = h->bucket_target;
   c->clean = h->clean;
   c->preserve_host = h->preserve_host;
   if ((rc = upload_preserve(c))) {

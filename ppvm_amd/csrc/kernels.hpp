@@ -438,9 +438,16 @@ static_assert(kSortBins <= kPassMax && kPassMax * 8 <= kLdsSlots * 4,
               "the sort-based pairing keeps its counters in cand and its sorted entries in tab");
 constexpr u32 kPosLimit = (1u << 20) - 1;     // bucket positions held by a slot (20 bits)
 
-__device__ __forceinline__ u32 pair_bucket_lg(u64 n_wl, u32 lg_max) {
+// Bucket count of a gate's partition: the smallest 2^lg (lg <= lg_max) whose
+// buckets hold at most `target` records on average (target = kBucketTarget
+// unless a test lowers it to reach the many-bucket branches).
+struct PartSpec {
+  u32 lg_max;
+  u32 target;
+};
+__device__ __forceinline__ u32 pair_bucket_lg(u64 n_wl, PartSpec ps) {
   u32 lg = 0;
-  while (lg < lg_max && ((u64)kBucketTarget << lg) < n_wl) ++lg;
+  while (lg < ps.lg_max && ((u64)ps.target << lg) < n_wl) ++lg;
   return lg;
 }
 __device__ __forceinline__ u32 bucket_of(u64 h, u32 lg) { return lg ? (u32)(h >> (64u - lg)) : 0u; }
@@ -579,7 +586,7 @@ __global__ __launch_bounds__(kBlock) void k_rot_pair(u64 cap, const u64* __restr
 // Slice s of the worklist counts its records per bucket: hist[s * B + b].
 // Also raises the overflow flag when the gate's children might not fit.
 __global__ __launch_bounds__(kPartThreads) void k_pair_hist(const u64* __restrict__ hsh, DevState* st,
-                                                           u32 in_slot, u64 cap, u32 lg_max, u64 op_tag,
+                                                           u32 in_slot, u64 cap, PartSpec ps, u64 op_tag,
                                                            u32* __restrict__ hist) {
   __shared__ u32 h[kMaxBuckets];
   if (st->overflow) return;
@@ -589,7 +596,7 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_hist(const u64* __restric
     if (blockIdx.x == 0 && threadIdx.x == 0) cas_u64(&st->overflow, 0ull, op_tag);
     return;
   }
-  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  const u32 lg = pair_bucket_lg(n_wl, ps), nb = 1u << lg;
   for (u32 b = threadIdx.x; b < nb; b += kPartThreads) h[b] = 0u;
   __syncthreads();
   const u64 lo = n_wl * blockIdx.x / kPairSlices, hi = n_wl * (blockIdx.x + 1) / kPairSlices;
@@ -610,14 +617,14 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_hist(const u64* __restric
 constexpr u32 kScanGroups = kPartThreads / 64;
 constexpr u32 kSlicesPerGroup = kPairSlices / kScanGroups;
 __global__ __launch_bounds__(kPartThreads) void k_pair_colscan(const DevState* __restrict__ st, u32 in_slot,
-                                                              u64 cap, u32 lg_max, u32* __restrict__ hist,
+                                                              u64 cap, PartSpec ps, u32* __restrict__ hist,
                                                               u32* __restrict__ bmeta) {
   __shared__ u32 gsum[kScanGroups][64];
   if (st->overflow) return;
   const u64 n_entries = st->slot[in_slot].n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
-  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  const u32 lg = pair_bucket_lg(n_wl, ps), nb = 1u << lg;
   const u32 b0 = blockIdx.x * 64u;
   if (b0 >= nb) return;
   const u32 lane = threadIdx.x & 63u, grp = threadIdx.x >> 6;
@@ -703,7 +710,7 @@ constexpr u32 kTiledBuckets = 1u << kTiledLgMax;
 constexpr u32 kTiledPer = kTiledBuckets / kPartThreads;
 __global__ __launch_bounds__(kPartThreads) void k_pair_scatter_tiled(const u64* __restrict__ hsh,
                                                                     const DevState* __restrict__ st, u32 in_slot,
-                                                                    u64 cap, u32 lg_max,
+                                                                    u64 cap, PartSpec ps,
                                                                     const u32* __restrict__ hist,
                                                                     u64* __restrict__ bkt, u32* __restrict__ bmeta) {
   __shared__ u32 off[kTiledBuckets];     // next global position per bucket (this slice)
@@ -717,7 +724,7 @@ __global__ __launch_bounds__(kPartThreads) void k_pair_scatter_tiled(const u64* 
   const u64 n_entries = st->slot[in_slot].n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
-  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  const u32 lg = pair_bucket_lg(n_wl, ps), nb = 1u << lg;
   if (lg > kTiledLgMax) {
     // too many buckets for the LDS tile: one 4-byte counter per bucket in
     // the tile buffer (kScatterTile u64 = kMaxBuckets u32) and a scattered
@@ -852,7 +859,7 @@ __device__ __forceinline__ u64 lds_probe_sync(const u64* __restrict__ keys, WMRe
 template <int W, bool kSort>
 __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap, const u64* __restrict__ keys,
                                                            WMRec* __restrict__ wm, DevState* st,
-                                                           u32 in_slot, RotSpec g, u32 lg_max,
+                                                           u32 in_slot, RotSpec g, PartSpec ps,
                                                            const u64* __restrict__ hsh,
                                                            const u64* __restrict__ bkt,
                                                            const u32* __restrict__ bmeta, u64* table,
@@ -870,7 +877,7 @@ __global__ __launch_bounds__(kPairThreads) void k_pair_lds(u64 cap, const u64* _
   const u64 n_entries = st->slot[in_slot].n_entries;
   const u64 n_wl = st->n_wl;
   if (n_entries + n_wl > cap) return;
-  const u32 lg = pair_bucket_lg(n_wl, lg_max), nb = 1u << lg;
+  const u32 lg = pair_bucket_lg(n_wl, ps), nb = 1u << lg;
   const u64 tmask = pair_table_mask(n_wl, tcap_max);
   const u32 tid = threadIdx.x;
   uint4* t4 = reinterpret_cast<uint4*>(tab);

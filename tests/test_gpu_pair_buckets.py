@@ -82,3 +82,39 @@ def test_single_bucket_branches(m, method, monkeypatch):
         assert fb >= total, f"only {fb} of {total} records took the global table"
     elif total > SUB_BINS * PASS_MAX:
         assert fb > 0
+
+
+@pytest.mark.parametrize("method", ["hash", "sort"])
+@pytest.mark.parametrize("target", [16, 64])
+def test_many_buckets_per_record_scatter(target, method, monkeypatch):
+    """With buckets aimed at 16 or 64 records, 300k records spread over 2^14 or
+    2^13 buckets: more than the tiled scatter's 2^12, so the scatter stores
+    record by record, and each LDS pass holds a small bucket."""
+    from oracle.pyoracle import OracleSum
+    from ppvm_amd import PauliSum
+    monkeypatch.setenv("PPVM_PARTITION_MIN_ENTRIES", "0")
+    monkeypatch.setenv("PPVM_SMALL_MAX_ENTRIES", "0")
+    monkeypatch.delenv("PPVM_PARTITION_MAX_LG", raising=False)
+    monkeypatch.setenv("PPVM_PARTITION_BUCKET_RECORDS", str(target))
+    if method == "sort":
+        monkeypatch.setenv("PPVM_PAIR_METHOD", "sort")
+    else:
+        monkeypatch.delenv("PPVM_PAIR_METHOD", raising=False)
+    rng = np.random.default_rng(4242 + target)
+    x, z, c = _words(rng, 200_000)
+    total = len(c)
+    assert total // target > 4096  # more buckets than the tiled scatter takes
+    first = "I" * N
+    g = PauliSum([first], N, coefficients=[0.5], min_abs_coeff=0.0)
+    r = OracleSum(N, 0.0, None)
+    r.add_strings([first], [0.5])
+    g._h.add_words(x, z, c)
+    r.add_words(x, z, c)
+    p = O.Program()
+    p.add(O.RZ, 0, 0, 0.3)
+    p.add(O.RX, 1, 0, 0.7)
+    p.add(O.RZZ, 2, 3, -0.4)
+    p.add(O.RZ, 0, 0, 1.1)
+    prog = p.array()
+    run_segments(g, r, prog, list(range(len(prog))), ctx=f"{total} records, {target} per bucket, {method}")
+    assert g._h.counters()["pair_fallback"] == 0
