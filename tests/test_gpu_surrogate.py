@@ -1,21 +1,20 @@
 """What the learned surrogate gains over the plain propagator sweep, timed live.
 
 The sweep here is the lattice TFIM grid of the surrogate workload restricted
-to t <= 1 (64 h x 32 t, steps <= 20), so that the plain sweep fits a test:
-the ensemble is fitted on the GPU on 1024 seed-31 training points of the
-shipped grid less a validation split of 20% of their (h, steps) pairs, the
-fallback bound on the ensemble std is calibrated on that split, and the other
-1024 points are queried (known pairs by lookup, new pairs by the ensemble
-under the bound, the propagator above it).  The plain sweep of the same query
-points runs live in the same test; its values judge every error and must
-reproduce the shipped grid (same term sets; overlaps within 1e-10).
+to t <= 0.75 (64 h x 24 t, steps <= 15), so that the plain sweep fits a test:
+the ensemble is fitted on the GPU on half of those points (seed-31 sample of
+the shipped grid) less a validation split of 20% of their (h, steps) pairs,
+the fallback bound on the ensemble std is calibrated on that split, and the
+other half is queried (known pairs by lookup, new pairs by the ensemble under
+the bound, the propagator above it).  The plain sweep of the same query points
+runs live in the same test; its values judge every error and must reproduce
+the shipped grid (same term sets; overlaps within 1e-10).
 
 Measured on one MI355X: the calibrated bound sends every new pair to the
-propagator (the ensemble agrees on points where the propagator's output jumps
-by more than 1e-3 between neighbouring h values), so at |err| <= 1e-3 the
-query phase gains only what the known pairs save (1.1x).  At the fixed std
-bound of 1e-2 the gain is 12x, with 88% of the points within 1e-3 and the
-rest within 1e-2.
+propagator (no validation point stays within 1e-3 under any std bound), so at
+|err| <= 1e-3 the query phase gains only what the known pairs save (1.05x).
+At the fixed std bound of 1e-2 no point falls back: the query is inference
+alone, 92% of the points within 1e-3 and all within 6.2e-3.
 """
 
 import numpy as np
@@ -30,12 +29,16 @@ def _propagate(h, m):
     return ser, secs
 
 
+T_MAX = 0.75  # 64 h x 24 t (steps <= 15): the plain sweep fits a test
+
+
 @pytest.mark.timeout(600)
 def test_surrogate_sweep_against_live_plain_sweep():
     from ppvm_amd.surrogate import load_grid, run_sweep
     d = load_grid()
-    tsel = d["t"] <= 1.0
-    rep = run_sweep(d["grid"], d["h"], d["t"][tsel], _propagate, n_train=1024, steps=12000, lr=2e-3,
+    tsel = d["t"] <= T_MAX
+    n_pts = len(d["h"]) * int(tsel.sum())
+    rep = run_sweep(d["grid"], d["h"], d["t"][tsel], _propagate, n_train=n_pts // 2, steps=12000, lr=2e-3,
                     log=lambda m: print(m, flush=True))
     print(f"bound {rep.bound:.2e}: known {rep.n_known} network {rep.n_accepted} fallback {rep.n_fallback} "
           f"({rep.fallback_h} h); frac<=1e-3 {rep.frac_within_bar:.4f} max {rep.max_err:.2e}; "
@@ -43,7 +46,7 @@ def test_surrogate_sweep_against_live_plain_sweep():
           f"validation {rep.val_accepted}/{rep.val_points}", flush=True)
     for row in rep.tradeoff:
         print(row, flush=True)
-    assert rep.n_query == 1024 and rep.n_known + rep.n_new == 1024
+    assert rep.n_query == n_pts - n_pts // 2 and rep.n_known + rep.n_new == rep.n_query
     # the training grid is this propagator's own output (same term sets; the
     # overlap sum may associate differently between engine versions)
     assert rep.grid_max_diff <= 1e-10
