@@ -807,10 +807,12 @@ u64 scan_chunk(u64 n) {
 }
 u32 scan_blocks(u64 n, u64 chunk) { return (u32)std::max<u64>(1, (n + chunk - 1) / chunk); }
 
-// Block sums (already in s->bsum) -> exclusive block offsets; returns the total.
+// Block sums (already in s->bsum) -> exclusive block offsets; returns the
+// total (read back to the host, one synchronisation) unless total is null.
 int lscan_offsets(ppvm_lossy_sum* s, u32 G, u64* total) {
   hipLaunchKernelGGL(k_lz_bscan, dim3(1), dim3(1024), 0, s->stream, s->bsum, G, s->dtot);
   LHIPCHK(hipGetLastError());
+  if (!total) return PPVM_OK;
   LHIPCHK(hipMemcpyAsync(s->htot, s->dtot, 8, hipMemcpyDeviceToHost, s->stream));
   LHIPCHK(hipStreamSynchronize(s->stream));
   *total = *s->htot;
@@ -957,9 +959,8 @@ int run_truncate(ppvm_lossy_sum* s) {
   const u64 tot = *s->htot;
   const u64 kept = tot & 0xffffffffull, restored = tot >> 32;
   if (kept == n) return PPVM_OK;  // nothing dropped
-  u64 check = 0;
   int rc;
-  if ((rc = lscan_offsets(s, G, &check))) return rc;
+  if ((rc = lscan_offsets(s, G, nullptr))) return rc;  // the kept total is already known
   hipLaunchKernelGGL(k_lz_scan<u64>, dim3(G), dim3(kLB), 0, s->stream, s->off, n, chunk, s->bsum, s->off2);
   const int o = s->cur ^ 1;
   hipLaunchKernelGGL(k_lz_compact<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, s->planes[s->cur], s->coef[s->cur],
