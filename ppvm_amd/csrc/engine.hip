@@ -1186,10 +1186,14 @@ bool small_eligible(const ppvm_sum* s) {
 }
 
 int small_buffers(ppvm_sum* s) {
-  if (s->d_small) return PPVM_OK;
-  if (hipMalloc(&s->d_small, kSmallBufferBytes) != hipSuccess ||
-      hipHostMalloc(&s->h_small, kSmallBufferBytes, hipHostMallocDefault) != hipSuccess) {
+  if (s->d_small && s->h_small) return PPVM_OK;
+  if ((!s->d_small && hipMalloc(&s->d_small, kSmallBufferBytes) != hipSuccess) ||
+      (!s->h_small && hipHostMalloc(&s->h_small, kSmallBufferBytes, hipHostMallocDefault) != hipSuccess)) {
     (void)hipGetLastError();
+    if (s->d_small) (void)hipFree(s->d_small);
+    if (s->h_small) (void)hipHostFree(s->h_small);
+    s->d_small = nullptr;
+    s->h_small = nullptr;
     return fail(PPVM_ENOMEM, "allocation of the small-run step buffers failed");
   }
   return PPVM_OK;
@@ -1620,6 +1624,9 @@ int create_impl(uint32_t n_qubits, double tau, uint64_t maxw, uint64_t cap_hint,
     delete s;
     return rc;
   }
+  // the LDS path's step buffers come with the handle, so that its first run
+  // does not pay for the allocation; without them the path stays off
+  if (!s->small_off && small_buffers(s) != PPVM_OK) s->small_off = true;
   *out = s;
   return PPVM_OK;
 }
