@@ -556,6 +556,7 @@ int launch_small(ppvm_sum* s, const Issued& it, u32 n_steps) {
   SmallArgs a{};
   a.keys = s->keys[s->cur];
   a.coef = s->coef[s->cur];
+  a.arena_cap = s->cap;
   a.st = s->st;
   a.in_slot = in;
   a.out_slot = out;

@@ -46,6 +46,7 @@ constexpr u32 kSmallStage = 32;  // step records staged in LDS at a time
 struct SmallArgs {
   u64* keys;                // arena (array of structures, 2W words per entry)
   double* coef;
+  u64 arena_cap;            // entries the arena holds: the run stops before a step that would exceed it
   DevState* st;
   u32 in_slot, out_slot;
   const SmallStep* steps;
@@ -231,10 +232,10 @@ __device__ __forceinline__ u32 small_slot(const u64 (&r)[2 * W]) {
 // below the index being read).  Returns 1 when the result is in Q/D,
 // 0 when no entry anticommutes with the generator (nothing changes: on a
 // clean state a fused truncation drops nothing either, so P/C stays
-// current), and -1 when the result would exceed CAP entries (P/C untouched).
+// current), and -1 when the result would exceed lim <= CAP entries (P/C untouched).
 template <int W, u32 CAP, u32 TS>
 __device__ __forceinline__ int small_rotate(const u64* P, const double* C, u64* Q, double* D, u32* table,
-                                            u32* partner, u32* wcnt, u32& par, u32& n, const RotSpec& g,
+                                            u32* partner, u32* wcnt, u32& par, u32& n, u32 lim, const RotSpec& g,
                                             const TruncSpec& t, u32 gen, SmallTally& ty) {
   const u32 T = blockDim.x, tid = threadIdx.x;
   const u32 tag = gen << 16;
@@ -304,7 +305,7 @@ __device__ __forceinline__ int small_rotate(const u64* P, const double* C, u64* 
     ty.children += keepc ? 1u : 0u;
     u32 p0, p1, t0, t1;
     small_rank2(keep, keepc, p0, p1, t0, t1, wcnt, par);
-    if (t0 + t1 > CAP) return -1;  // uniform
+    if (t0 + t1 > lim) return -1;  // uniform
     if (keep) {
       sstore<W, CAP>(Q, p0, k);
       D[p0] = c;
@@ -329,7 +330,7 @@ __device__ __forceinline__ int small_rotate(const u64* P, const double* C, u64* 
       ty.children += (pass == 1 && keep) ? 1u : 0u;
       u32 pos;
       const u32 tot = small_rank(keep, pos, wcnt, par);
-      if (out0 + out1 + tot > CAP) return -1;  // uniform
+      if (out0 + out1 + tot > lim) return -1;  // uniform
       if (keep) {
         const u32 d = out0 + (pass ? out1 : 0u) + pos;
         sstore<W, CAP>(Q, d, k);
@@ -401,6 +402,7 @@ __global__ __launch_bounds__(kSmallMaxThreads) void k_small(SmallArgs a) {
   if (st->overflow) return;
   const u32 T = blockDim.x, tid = threadIdx.x;
   const CountSlot in = st->slot[a.in_slot];
+  const u32 lim = a.arena_cap < CAP ? (u32)a.arena_cap : CAP;  // entries a step may leave
   for (u32 j = tid; j < TS; j += T) table[j] = 0u;
   for (u32 j = tid; j < CAP; j += T) partner[j] = 0u;
   // live entries, in order
@@ -412,7 +414,7 @@ __global__ __launch_bounds__(kSmallMaxThreads) void k_small(SmallArgs a) {
     const bool live = i < in.n_entries && !is_dead(c = a.coef[i]);
     u32 pos;
     const u32 tot = small_rank(live, pos, wcnt, par);
-    if (n + tot > CAP) {
+    if (n + tot > lim) {
       fits = false;
       break;
     }
@@ -463,7 +465,7 @@ __global__ __launch_bounds__(kSmallMaxThreads) void k_small(SmallArgs a) {
         gen = 1;
       }
       const int r = small_rotate<W, CAP, TS>(lk[cur], lc[cur], lk[cur ^ 1], lc[cur ^ 1], table, partner, wcnt,
-                                             par, n, g, a.t, gen, ty);
+                                             par, n, lim, g, a.t, gen, ty);
       if (r < 0) break;
       cur ^= (u32)r;
       uni_updates += n_in;

@@ -37,10 +37,14 @@ def test_small_configs_run_in_lds(wname, threads, monkeypatch):
     assert c["map_launches"] == 0 and c["transposed"] == 0  # no arena-pipeline kernel ran
 
 
-def test_small_hands_over_when_the_sum_outgrows_lds():
+@pytest.mark.parametrize("whole", [False, True])
+def test_small_hands_over_when_the_sum_outgrows_lds(whole):
+    """Four lattice steps (3,588 terms), applied layer by layer or as one
+    program (one run from a single term, handing over mid-program)."""
     w = WL.truncate_steps(WL.lattice_tfim(), 4)
     g, r = gpu_for(w), oracle_for(w)
-    run_segments(g, r, w.program, w.layer_marks, ctx="lattice 4 steps")
+    marks = [len(w.program) - 1] if whole else w.layer_marks
+    run_segments(g, r, w.program, marks, ctx=f"lattice 4 steps whole {whole}")
     c = g._h.counters()
     assert c["term_updates"] == r.term_updates()
     assert c["small_runs"] >= 1 and c["transposed"] > 0  # started in LDS, finished in the arena
