@@ -778,6 +778,17 @@ def _short_kernel(name: str) -> str:
     return m.group(1) if m else name.split("(")[0][-80:]
 
 
+def _last_gates(rows: list, last_gates: int) -> list:
+    """The dispatches (sorted rows whose second field is the short kernel name)
+    of the last `last_gates` rotation gates: from the k_rot_count before the
+    last_gates-th last k_rot_gather on."""
+    gi = [i for i, r in enumerate(rows) if r[1] == "k_rot_gather"]
+    start = gi[-last_gates] if len(gi) >= last_gates else 0
+    while start > 0 and rows[start][1] != "k_rot_count":
+        start -= 1
+    return rows[start:]
+
+
 def counter_rows(root: str, counter: str, last_gates: int = 0) -> dict:
     """kernel -> (dispatches, mean counter value) from the counter_collection
     CSVs rocprofv3 wrote under root; with last_gates > 0 only the dispatches of
@@ -794,11 +805,7 @@ def counter_rows(root: str, counter: str, last_gates: int = 0) -> dict:
                     rows.append((int(r["Dispatch_Id"]), _short_kernel(r["Kernel_Name"]), float(r["Counter_Value"])))
     rows.sort()
     if last_gates:
-        gi = [i for i, r in enumerate(rows) if r[1] == "k_rot_gather"]
-        start = gi[-last_gates] if len(gi) >= last_gates else 0
-        while start > 0 and rows[start][1] != "k_rot_count":
-            start -= 1
-        rows = rows[start:]
+        rows = _last_gates(rows, last_gates)
     acc = defaultdict(lambda: [0, 0.0])
     for _, k, v in rows:
         acc[k][0] += 1
@@ -829,9 +836,33 @@ def run_traffic(args) -> dict:
 
 def run_kstats(args) -> dict:
     """Per-kernel statistics from a rocprofv3 --kernel-trace --stats run
-    (the *kernel_stats.csv under --dir), durations in microseconds."""
+    (the *kernel_stats.csv under --dir), durations in microseconds; with
+    --last-gates N, from the kernel trace of the last N rotation gates only."""
     import csv
     import glob
+    if args.last_gates:
+        tr = []
+        for f in sorted(glob.glob(os.path.join(args.dir, "**", "*kernel_trace.csv"), recursive=True)):
+            with open(f, newline="") as fh:
+                for r in csv.DictReader(fh):
+                    tr.append((int(r["Start_Timestamp"]), _short_kernel(r["Kernel_Name"]),
+                               int(r["End_Timestamp"])))
+        tr.sort()
+        tr = _last_gates(tr, args.last_gates)
+        acc = {}
+        for t0, k, t1 in tr:
+            d = (t1 - t0) / 1e3
+            a = acc.setdefault(k, [0, 0.0, float("inf"), 0.0])
+            a[0] += 1
+            a[1] += d
+            a[2] = min(a[2], d)
+            a[3] = max(a[3], d)
+        busy = sum(a[1] for a in acc.values())
+        rows = [{"kernel": k, "calls": a[0], "total_ms": a[1] / 1e3, "avg_us": a[1] / a[0],
+                 "pct": 100.0 * a[1] / busy, "min_us": a[2], "max_us": a[3]} for k, a in acc.items()]
+        rows.sort(key=lambda r: -r["total_ms"])
+        return {"metric": "rocprofv3 kernel statistics", "kernels": rows, "last_gates": args.last_gates,
+                "span_us": (tr[-1][2] - tr[0][0]) / 1e3 if tr else 0.0, "busy_us": busy}
     rows = []
     for f in sorted(glob.glob(os.path.join(args.dir, "**", "*kernel_stats.csv"), recursive=True)):
         with open(f, newline="") as fh:
@@ -850,6 +881,9 @@ def run_kstats(args) -> dict:
 def print_table(res: dict) -> None:
     """Text table of a kstats or traffic result (what profiles/ holds)."""
     if res["metric"].startswith("rocprofv3 kernel"):
+        if res.get("last_gates"):
+            print(f"# last {res['last_gates']} rotation gates: span {res['span_us']:.0f} us, "
+                  f"kernels busy {res['busy_us']:.0f} us")
         print(f"{'kernel':<26} {'calls':>7} {'total_ms':>10} {'avg_us':>9} {'pct':>6} {'min_us':>8} {'max_us':>9}")
         for r in res["kernels"]:
             print(f"{r['kernel']:<26} {r['calls']:>7d} {r['total_ms']:>10.2f} {r['avg_us']:>9.2f} {r['pct']:>6.2f} "
@@ -873,7 +907,8 @@ def main() -> int:
     ap.add_argument("--cal-write", help="traffic: ... calibration WRITE_SIZE run")
     ap.add_argument("--fetch", help="traffic: ... workload FETCH_SIZE run")
     ap.add_argument("--write", help="traffic: ... workload WRITE_SIZE run")
-    ap.add_argument("--last-gates", type=int, default=0, help="traffic: only the last N rotation gates")
+    ap.add_argument("--last-gates", type=int, default=0,
+                    help="traffic, kstats: only the last N rotation gates (kstats reads the kernel trace)")
     ap.add_argument("--verify", type=int, default=16, help="batched: instances re-run on the engine")
     ap.add_argument("--cpu-instances", type=int, default=256, help="batched: at most this many instances on the CPU")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="batched: CPU baseline time budget (s)")
