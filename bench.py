@@ -921,8 +921,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU runs are not part of "
-                                   "this build (--gpus must be 1)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this engine runs on one GPU; sharding a sum across GPUs is not "
+                                   "implemented (--gpus must be 1)", "n_gpus": args.gpus}))
         return 2
     res = {"lattice": run, "batched": run_batched, "surrogate": run_surrogate,
            "parity": run_parity, "lossy": run_lossy, "example": run_example,
