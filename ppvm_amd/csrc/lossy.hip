@@ -725,6 +725,11 @@ struct ppvm_lossy_sum {
   double* partials = nullptr;
   u64* preserve = nullptr;
   u32 n_preserve = 0;
+  // Local states some term may hold on each qubit (a superset, widened by
+  // every op that runs): bit q of smask[s] for state s (I X Z Y L = 0..4).
+  // An op whose rule table leaves every such state unchanged is skipped.
+  std::vector<u64> smask[5];
+  bool clean = true;  // a truncation ran after the last change (another one drops nothing)
 };
 
 namespace {
@@ -1353,6 +1358,73 @@ int check_words(const ppvm_lossy_sum* s, const uint64_t* x, const uint64_t* z, c
 
 extern "C" {
 
+namespace {
+
+// Local-state masks of a list of words (np = 3W words each: x, z, l planes).
+void lstates_of(ppvm_lossy_sum* s, const std::vector<u64>& keys, size_t count) {
+  const u32 W = s->W, Wa = s->Wa;
+  for (auto& m : s->smask) m.assign(Wa, 0ull);
+  for (size_t i = 0; i < count; ++i) {
+    const u64* k = keys.data() + i * 3 * W;
+    for (u32 p = 0; p < Wa; ++p) {
+      const u64 x = k[p], z = k[W + p], l = k[2 * W + p];
+      const u64 valid = p + 1 < Wa || s->n % 64 == 0 ? ~0ull : (1ull << (s->n % 64)) - 1ull;
+      s->smask[0][p] |= ~(x | z | l) & valid;
+      s->smask[1][p] |= x & ~z;
+      s->smask[2][p] |= z & ~x;
+      s->smask[3][p] |= x & z;
+      s->smask[4][p] |= l;
+    }
+  }
+}
+
+u32 lstates_at(const ppvm_lossy_sum* s, u32 q) {
+  u32 m = 0;
+  for (u32 t = 0; t < 5; ++t) m |= (u32)((s->smask[t][q / 64] >> (q % 64)) & 1ull) << t;
+  return m;
+}
+
+// Whether the op's rule table leaves every local state the sum may hold on
+// its qubits unchanged (factor 1, same state, no child): then it changes nothing.
+bool lop_is_identity(const ppvm_lossy_sum* s, const LTable& tb) {
+  const u32 ma = lstates_at(s, tb.a), mb = tb.two ? lstates_at(s, tb.b) : 1u;
+  for (u32 sa = 0; sa < 5; ++sa) {
+    if (!((ma >> sa) & 1u)) continue;
+    for (u32 sb = 0; sb < (tb.two ? 5u : 1u); ++sb) {
+      if (!((mb >> sb) & 1u)) continue;
+      const u32 t = tb.two ? sa * 5u + sb : sa;
+      const LEntry& e = tb.e[t];
+      if (e.pf != 1.0 || e.pst != t || e.nch != 0) return false;
+    }
+  }
+  return true;
+}
+
+// Widens the masks by the states an op that ran can leave on its qubits.
+void lstates_after(ppvm_lossy_sum* s, const LTable& tb) {
+  const u32 ma = lstates_at(s, tb.a), mb = tb.two ? lstates_at(s, tb.b) : 1u;
+  u32 na = ma, nb = mb;
+  for (u32 sa = 0; sa < 5; ++sa) {
+    if (!((ma >> sa) & 1u)) continue;
+    for (u32 sb = 0; sb < (tb.two ? 5u : 1u); ++sb) {
+      if (!((mb >> sb) & 1u)) continue;
+      const LEntry& e = tb.e[tb.two ? sa * 5u + sb : sa];
+      u32 outs[4] = {e.pst, 0, 0, 0};
+      for (u32 j = 0; j < e.nch; ++j) outs[1 + j] = e.cst[j];
+      for (u32 j = 0; j <= e.nch; ++j) {
+        na |= 1u << (tb.two ? outs[j] / 5u : outs[j]);
+        if (tb.two) nb |= 1u << (outs[j] % 5u);
+      }
+    }
+  }
+  for (u32 t = 0; t < 5; ++t) {
+    if ((na >> t) & 1u) s->smask[t][tb.a / 64] |= 1ull << (tb.a % 64);
+    if (tb.two && ((nb >> t) & 1u)) s->smask[t][tb.b / 64] |= 1ull << (tb.b % 64);
+  }
+}
+
+}  // namespace
+
 const char* ppvm_lossy_last_error(void) { return l_err.c_str(); }
 
 int ppvm_lossy_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_pauli_weight, uint64_t max_loss_weight,
@@ -1389,6 +1461,7 @@ int ppvm_lossy_create(uint32_t n_qubits, double min_abs_coeff, uint64_t max_paul
     ppvm_lossy_destroy(s);
     return rc;
   }
+  lstates_of(s, {}, 0);
   *out = s;
   return PPVM_OK;
 }
@@ -1440,6 +1513,8 @@ int ppvm_lossy_clone(ppvm_lossy_handle s, ppvm_lossy_handle* out) {
   }
   LHIPCHK(hipStreamSynchronize(c->stream));
   c->len = s->len;
+  for (int t = 0; t < 5; ++t) c->smask[t] = s->smask[t];
+  c->clean = s->clean;
   *out = c;
   return PPVM_OK;
 }
@@ -1463,6 +1538,7 @@ int ppvm_lossy_set_preserve(ppvm_lossy_handle s, const uint64_t* x, const uint64
   if (s->preserve) (void)hipFree(s->preserve);
   s->preserve = nullptr;
   s->n_preserve = 0;
+  s->clean = false;
   if (!count) return PPVM_OK;
   if (hipMalloc(&s->preserve, keys.size() * 8) != hipSuccess) {
     (void)hipGetLastError();
@@ -1503,6 +1579,8 @@ int ppvm_lossy_add_terms(ppvm_lossy_handle s, const uint64_t* x, const uint64_t*
       cs.push_back(c[i]);
     }
   }
+  lstates_of(s, keys, cs.size());
+  s->clean = false;
   return broken(s, lupload(s, keys, cs));
 }
 
@@ -1516,12 +1594,19 @@ int ppvm_lossy_apply(ppvm_lossy_handle s, const ppvm_op* ops, size_t n) {
     const ppvm_op& op = ops[i];
     if (op.code == PPVM_OP_NOP) continue;
     if (op.code == PPVM_OP_TRUNCATE) {
+      // on a state a truncation already left (nothing changed since) it drops
+      // nothing and moves nothing
+      if (s->clean) continue;
       if ((rc = run_truncate_w(s))) return broken(s, rc);
+      s->clean = true;
       continue;
     }
     LTable tb;
     if ((rc = build_table(op, tb))) return rc;
+    if (lop_is_identity(s, tb)) continue;  // acts on no local state a term holds
     if ((rc = run_op_w(s, tb))) return broken(s, rc);
+    lstates_after(s, tb);
+    s->clean = false;
   }
   LHIPCHK(hipStreamSynchronize(s->stream));
   return PPVM_OK;

@@ -131,6 +131,51 @@ def test_preserve_and_overlap_and_clone():
     assert abs(a.overlap(b) - oa.overlap(ob)) <= 1e-10
 
 
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_ops_that_change_nothing_are_skipped_exactly(seed):
+    """The engine skips an op whose rule table leaves every local state the
+    sum may hold on its qubits unchanged, and a truncation with nothing
+    changed since the last one.  Words live on qubits 0-5 (with lost qubits),
+    and the program mixes ops on qubits no term touches, rotations on qubits
+    holding only I and L, zero-probability noise, repeated truncations of a
+    preserve set whose words fall below the threshold, and Cliffords that move
+    states onto new qubits before rotations act there; the oracle runs every op."""
+    n = 20
+    rng = np.random.default_rng(500 + seed)
+    from ppvm_amd.ops import pack_lossy_strings
+    keep = ["ZLIIIIIIIIIIIIIIIIII", "IIXIIIIIIIIIIIIIIIII"]
+    g, o = _handle(n, 0.02, 6, 2), LossyOracleSum(n, 0.02, 6, 2)
+    g.set_preserve(*pack_lossy_strings(keep, n))
+    o.set_preserve(keep)
+    words = _words(rng, n, 30, np.arange(6)) + keep
+    _load(g, o, words, list(rng.uniform(-1, 1, len(words) - 2)) + [0.01, -0.015])
+    p = O.Program()
+    for _ in range(120):
+        r = rng.random()
+        if r < 0.25:  # qubits no term touches yet (until a Clifford moves a state there)
+            a, b = (int(v) for v in rng.choice(np.arange(10, 20), 2, replace=False))
+        else:
+            a, b = (int(v) for v in rng.choice(np.arange(0, 12), 2, replace=False))
+        k = int(rng.choice([O.RX, O.RY, O.RZ, O.RZZ, O.RXX, O.CNOT, O.H, O.PAULI_ERROR, O.LOSS, O.RESET_LOSS]))
+        probs = [0.0] * 15 if (k == O.PAULI_ERROR and rng.random() < 0.5) else list(rng.uniform(0.0, 0.05, 15))
+        p.add(k, a, b, float(rng.uniform(-2, 2)), probs)
+        for _ in range(int(rng.integers(0, 3))):
+            p.truncate()
+    prog = p.array()
+    for s in range(0, len(prog), 15):
+        g.apply(prog[s:s + 15])
+        o.apply(prog[s:s + 15])
+        _same(g, o, f"seed {seed} ops {s}")
+    c, oc = g.clone(), o.copy()
+    tail = O.Program()
+    tail.truncate()
+    tail.add(O.RZ, 15, 0, 0.4)
+    tail.truncate()
+    c.apply(tail.array())
+    oc.apply(tail.array())
+    _same(c, oc, "clone")
+
+
 def test_python_lossy_known_answers():
     from ppvm_amd import LossyPauliSum
     p = 0.1
