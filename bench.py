@@ -867,7 +867,7 @@ def run_kstats(args) -> dict:
     for f in sorted(glob.glob(os.path.join(args.dir, "**", "*kernel_stats.csv"), recursive=True)):
         with open(f, newline="") as fh:
             for r in csv.DictReader(fh):
-                name = r["Name"]
+                name = r["Name"].replace("(anonymous namespace)", "")
                 k = _short_kernel(name)
                 if "<" in name.split("(")[0] and k.startswith("k_"):
                     k += "<" + name.split("(")[0].split("<", 1)[1]
