@@ -214,11 +214,14 @@ __device__ __forceinline__ void block_total(u64 v, u64* dst) {
 }
 
 // Words of the map are distinct, so an insert never compares keys.  The index
-// has at least 2n slots, so every probe ends.
+// has at least twice as many slots as words inserted, so every probe ends.
+// With `only` set, just the entries with only[i] != 0 are inserted (the parents
+// of a branch op: the entries with children).
 template <int W>
 __global__ __launch_bounds__(kLB) void k_lz_index(const u64* __restrict__ planes, u64 cap, u64 n,
-                                                  u32* __restrict__ table, u64 tmask) {
+                                                  u32* __restrict__ table, u64 tmask, const u32* __restrict__ only) {
   for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) {
+    if (only && only[i] == 0u) continue;
     u64 k[3 * W];
     lload<W>(planes, cap, i, k);
     u64 pos = lhash<W>(k) & tmask;
@@ -286,11 +289,13 @@ __device__ __forceinline__ int gather(const u64* __restrict__ planes, u64 cap, c
 
 // Children pass 1: flag the children that create a new entry (word absent from
 // the map, and this child is its first contributor in (parent, slot) order).
+// Whether the word is in the map: present[child] as k_lz_merge marked it, or,
+// without `present`, a lookup in an index of every word.
 template <int W>
 __global__ __launch_bounds__(kLB) void k_lz_child_flags(const u64* __restrict__ planes, u64 cap, u64 n, LTable tb,
                                                         const u32* __restrict__ cnt, const u64* __restrict__ off,
                                                         const u32* __restrict__ table, u64 tmask,
-                                                        u32* __restrict__ flag) {
+                                                        const u32* __restrict__ present, u32* __restrict__ flag) {
   for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) {
     const u32 nch = cnt[i];
     if (nch == 0u) continue;
@@ -304,7 +309,7 @@ __global__ __launch_bounds__(kLB) void k_lz_child_flags(const u64* __restrict__ 
       for (int q = 0; q < 3 * W; ++q) T[q] = k[q];
       put_local<W>(T, tb, e.cst[j]);
       u32 f = 0u;
-      if (lfind<W>(planes, cap, table, tmask, T) == ~0ull) {
+      if (present ? present[base + j] == 0u : lfind<W>(planes, cap, table, tmask, T) == ~0ull) {
         Contrib c[3];
         const int m = gather<W>(planes, cap, table, tmask, tb, T, e.cst[j], c);
         f = (m > 0 && c[0].p == i && c[0].j == j) ? 1u : 0u;
@@ -347,10 +352,13 @@ __global__ __launch_bounds__(kLB) void k_lz_child_write(u64* planes, u64 cap, u6
 }
 
 // Existing entries that children land on: `+=` of the contributions in order.
+// With `present`, each contribution's child is marked there (its word is in
+// the map), at off[parent] + slot.
 template <int W>
 __global__ __launch_bounds__(kLB) void k_lz_merge(const u64* __restrict__ planes, u64 cap, u64 n, LTable tb,
                                                   const u32* __restrict__ table, u64 tmask,
-                                                  const double* __restrict__ cin, double* __restrict__ cout) {
+                                                  const double* __restrict__ cin, double* __restrict__ cout,
+                                                  const u64* __restrict__ off, u32* __restrict__ present) {
   for (u64 i = (u64)blockIdx.x * kLB + threadIdx.x; i < n; i += (u64)gridDim.x * kLB) {
     u64 k[3 * W];
     lload<W>(planes, cap, i, k);
@@ -359,6 +367,8 @@ __global__ __launch_bounds__(kLB) void k_lz_merge(const u64* __restrict__ planes
     Contrib c[3];
     const int m = gather<W>(planes, cap, table, tmask, tb, k, t, c);
     if (m == 0) continue;
+    if (present)
+      for (int r = 0; r < m; ++r) present[off[c[r].p] + c[r].j] = 1u;
     if (tb.multi) {
       // map_insert_multiple: the children meet in the aux map first, and their
       // sum is added to the entry once (sum/data.rs:234-247, consume :181-189)
@@ -840,14 +850,18 @@ int lscan(ppvm_lossy_sum* s, const T* in, u64 n, u64* out, u64* total) {
   return PPVM_OK;
 }
 
-// Index of n words in the first pow2(2n) slots of table (tcap slots); returns
-// the slot mask through *tmask.
+// Index of the first n words (or, with `only`, of those with only[i] != 0, at
+// most m of them) in the first pow2(2m) slots of table (tcap slots, m = n
+// without `only`); returns the slot mask through *tmask.
 template <int W>
-int lindex_w(ppvm_lossy_sum* s, const u64* planes, u64 cap, u64 n, u32* table, u64 tcap, u64* tmask) {
-  const u64 slots = std::min(tcap, lpow2(std::max<u64>(2 * n, 64)));
+int lindex_w(ppvm_lossy_sum* s, const u64* planes, u64 cap, u64 n, u32* table, u64 tcap, u64* tmask,
+             const u32* only = nullptr, u64 m = 0) {
+  const u64 slots = std::min(tcap, lpow2(std::max<u64>(2 * (only ? m : n), 64)));
   *tmask = slots - 1;
   hipLaunchKernelGGL(k_lz_fill, dim3(lgrid(slots)), dim3(kLB), 0, s->stream, table, slots);
-  if (n) hipLaunchKernelGGL(k_lz_index<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, planes, cap, n, table, slots - 1);
+  if (n)
+    hipLaunchKernelGGL(k_lz_index<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, planes, cap, n, table, slots - 1,
+                       only);
   LHIPCHK(hipGetLastError());
   return PPVM_OK;
 }
@@ -894,14 +908,29 @@ int run_op(ppvm_lossy_sum* s, const LTable& tb) {
   }
   hipLaunchKernelGGL(k_lz_scan<u32>, dim3(G), dim3(kLB), 0, s->stream, s->cnt, n, chunk, s->bsum, s->off);
   u64 tmask = 0;
-  if ((rc = lindex_w<W>(s, P, s->cap, n, s->table, s->tcap, &tmask))) return rc;
-  hipLaunchKernelGGL(k_lz_child_flags<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->cnt,
-                     s->off, s->table, tmask, s->flag);
-  LHIPCHK(hipGetLastError());
   u64 NEW = 0;
-  if ((rc = lscan<u32>(s, s->flag, M, s->pos, &NEW))) return rc;
-  hipLaunchKernelGGL(k_lz_merge<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->table, tmask,
-                     cin, cout);
+  if (!tb.multi) {
+    // index of the parents only (at most M words): every lookup below is of a
+    // parent; whether a child's word is in the map is marked by the merge,
+    // which finds each existing word's contributions
+    u32* present = s->flag2;
+    hipLaunchKernelGGL(k_lz_fill, dim3(lgrid(M)), dim3(kLB), 0, s->stream, present, M);
+    if ((rc = lindex_w<W>(s, P, s->cap, n, s->table, s->tcap, &tmask, s->cnt, M))) return rc;
+    hipLaunchKernelGGL(k_lz_merge<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->table, tmask,
+                       cin, cout, s->off, present);
+    hipLaunchKernelGGL(k_lz_child_flags<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->cnt,
+                       s->off, s->table, tmask, present, s->flag);
+    LHIPCHK(hipGetLastError());
+    if ((rc = lscan<u32>(s, s->flag, M, s->pos, &NEW))) return rc;
+  } else {
+    if ((rc = lindex_w<W>(s, P, s->cap, n, s->table, s->tcap, &tmask))) return rc;
+    hipLaunchKernelGGL(k_lz_child_flags<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->cnt,
+                       s->off, s->table, tmask, nullptr, s->flag);
+    LHIPCHK(hipGetLastError());
+    if ((rc = lscan<u32>(s, s->flag, M, s->pos, &NEW))) return rc;
+    hipLaunchKernelGGL(k_lz_merge<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->table, tmask,
+                       cin, cout, nullptr, nullptr);
+  }
   if (NEW)
     hipLaunchKernelGGL(k_lz_child_write<W>, dim3(lgrid(n)), dim3(kLB), 0, s->stream, P, s->cap, n, tb, s->cnt,
                        s->off, s->table, tmask, s->flag, s->pos, cin, cout);
