@@ -116,6 +116,11 @@ struct MapProgram {
   u32 trunc_before;        // full truncation check before the first op
   MapOp ops[kMaxMapOps];
   u32 apply_mask[kMaxFactorTables];   // 16-bit mask of table entries that apply
+  // dropped ops (identity on every term) that the program books as term
+  // updates: idle[j] of them ran before op j (idle[n_ops]: after the last),
+  // n_idle in all
+  u32 n_idle;
+  unsigned short idle[kMaxMapOps + 2];
   double factors[kMaxFactorTables][16];
 };
 

@@ -67,6 +67,7 @@ struct Issued {
   int kind;
   u64 seq;
   u32 nskip;     // IK_SKIP: rotations in the run
+  u32 nskipm;    // IK_SKIP: map ops in the run
   u32 mlo, mhi;  // IK_MAP collected for an LDS run: its source ops in small_mapops
   RotSpec rot;
   MapProgram map;
@@ -150,7 +151,8 @@ struct ppvm_sum {
   // Union of the x (and z) words of every term the sum may hold: exact after
   // an upload, then widened by every op that can set a bit, never narrowed.
   std::vector<u64> xsup, zsup;
-  u32 skip_run = 0;  // rotations dropped since the last issued op (see k_skip_gates)
+  u32 skip_run = 0;   // rotations dropped since the last issued op (see k_skip_gates)
+  u32 skip_maps = 0;  // map ops dropped since then (identity on every term)
   u32 tab_gen = 0;
   bool clean = true;
   bool broken = false;
@@ -521,7 +523,7 @@ int launch_map(ppvm_sum* s, const Issued& it) {
   fa.Ga = Gm;
   fa.aux0 = s->part(0);
   fa.aux1 = s->part(1);
-  fa.n_ops = it.map.n_ops;
+  fa.n_ops = it.map.n_ops + it.map.n_idle;
   if ((rc = launch_finish(s, fa))) return rc;
   HIPCHK(hipGetLastError());
   for (u32 i = 0; i < it.map.n_ops; ++i) {
@@ -757,7 +759,8 @@ int launch_skip(ppvm_sum* s, const Issued& it) {
   const u32 in = (u32)(it.seq & 1), out = in ^ 1u;
   int rc;
   if ((rc = prof_begin(s, PH_OTHER))) return rc;
-  hipLaunchKernelGGL(k_skip_gates, dim3(1), dim3(64), 0, s->stream, s->st, in, out, it.nskip);
+  hipLaunchKernelGGL(k_skip_gates, dim3(1), dim3(64), 0, s->stream, s->st, in, out, it.nskip + it.nskipm,
+                     it.nskip);
   if ((rc = prof_end(s))) return rc;
   HIPCHK(hipGetLastError());
   return PPVM_OK;
@@ -1110,6 +1113,14 @@ void sup_after_rot(ppvm_sum* s, const RotSpec& g) {
 }
 // Clifford bit maps applied to the support (a swap is exact, an XOR becomes an
 // OR); sign-only gates, noise and truncation leave it as it is.
+// A map op (Clifford or diagonal noise) on qubits where every term holds the
+// identity changes nothing: Cliffords map I to I with sign +, and every noise
+// factor of the identity is 1.
+bool map_is_identity(const ppvm_sum* s, const ppvm_op& op) {
+  if (sup_bit(s->xsup, op.q0) || sup_bit(s->zsup, op.q0)) return false;
+  return !is_two_qubit(op.code) || (!sup_bit(s->xsup, op.q1) && !sup_bit(s->zsup, op.q1));
+}
+
 void sup_after_map(ppvm_sum* s, const ppvm_op& op) {
   const u32 a = op.q0, b = op.q1;
   const bool xa = sup_bit(s->xsup, a), za = sup_bit(s->zsup, a);
@@ -1165,11 +1176,13 @@ int issue_pres(ppvm_sum* s) {
 
 // Issues the pending run of dropped rotations (their bookkeeping kernel).
 int flush_skips(ppvm_sum* s) {
-  if (!s->skip_run) return PPVM_OK;
+  if (!s->skip_run && !s->skip_maps) return PPVM_OK;
   Issued it{};
   it.kind = IK_SKIP;
   it.nskip = s->skip_run;
+  it.nskipm = s->skip_maps;
   s->skip_run = 0;
+  s->skip_maps = 0;
   return issue_and_maybe_sync(s, it);
 }
 
@@ -1228,6 +1241,7 @@ int small_flush(ppvm_sum* s) {
     } else {
       st.kind = SK_SKIP;
       st.arg = it.nskip;
+      st.arg2 = it.nskipm;
     }
   }
   const u32 n = (u32)list.size();
@@ -1284,6 +1298,7 @@ int small_flush(ppvm_sum* s) {
     Issued& it = list[k];
     if (it.kind == IK_SKIP) {
       s->skip_run += it.nskip;
+      s->skip_maps += it.nskipm;
       continue;
     }
     if (it.kind == IK_ROT && it.rot.mode != kBranchDamping && !rot_may_act(s, it.rot)) {
@@ -1417,6 +1432,17 @@ int pump_ops(ppvm_sum* s, bool final) {
         continue;
       }
       const bool tnext = k + 1 < q.size() && q[k + 1].code == PPVM_OP_TRUNCATE && s->trunc_active;
+      if (map_is_identity(s, m) && (!tnext || clean) && p.idle[p.n_ops] < 0xffffu) {
+        // dropped (a truncation after it would find the state it left, clean);
+        // the program books its term updates at its position
+        ++p.idle[p.n_ops];
+        ++p.n_idle;
+        ++k;
+        if (tnext) {
+          while (k < j && q[k].code == PPVM_OP_TRUNCATE) ++k;
+        }
+        continue;
+      }
       if (!map_add(p, ntables, m, tnext)) break;  // program full: issue and continue
       sup_after_map(s, m);
       if (s->small_on) s->small_mapops.push_back(m);
@@ -1431,6 +1457,11 @@ int pump_ops(ppvm_sum* s, bool final) {
       }
     }
     it.mhi = (u32)s->small_mapops.size();
+    if (!any && !p.trunc_before && p.n_idle) {
+      // only dropped ops: booked with the dropped gates (no truncation point
+      // among them, so they see the current live count)
+      s->skip_maps += p.n_idle;
+    }
     if (any || p.trunc_before) {
       if ((rc = flush_skips(s))) return rc;
       if ((rc = issue_and_maybe_sync(s, it))) return rc;
@@ -1804,6 +1835,7 @@ int ppvm_clear(ppvm_handle h) {
   // discard them (an overflow they raised needs no replay either)
   h->pending.clear();
   h->skip_run = 0;
+  h->skip_maps = 0;
   h->log.clear();
   h->issued_since_sync = 0;
   h->small_list.clear();

@@ -1306,19 +1306,20 @@ __global__ __launch_bounds__(kBlock) void k_fingerprint(const u64* __restrict__ 
   }
 }
 
-// ---- k_skip_gates: rotations that act on no term --------------------------------
+// ---- k_skip_gates: gates that act on no term -------------------------------------
 // The host drops a run of rotations whose generators anticommute with no Pauli
-// present anywhere in the sum (judged from the union of the terms' x and z
-// supports); such gates leave every term and coefficient unchanged.  This
-// kernel carries the counts over to the next op slot and books the term
-// updates the gates represent.
-__global__ void k_skip_gates(DevState* st, u32 in_slot, u32 out_slot, u32 n_gates) {
+// present anywhere in the sum, and of map ops on qubits where every term holds
+// the identity (judged from the union of the terms' x and z supports); such
+// gates leave every term and coefficient unchanged.  This kernel carries the
+// counts over to the next op slot and books the term updates the gates (n_rot
+// of them rotations) represent.
+__global__ void k_skip_gates(DevState* st, u32 in_slot, u32 out_slot, u32 n_gates, u32 n_rot) {
   if (threadIdx.x != 0 || st->overflow) return;
   const CountSlot in = st->slot[in_slot];
   st->slot[out_slot] = in;
   st->term_updates += (u64)n_gates * (in.n_entries - in.n_dead);
   st->gates += n_gates;
-  st->rotations += n_gates;
+  st->rotations += n_rot;
 }
 
 // ---- preserved strings that a truncation would drop ---------------------------------
@@ -1470,11 +1471,13 @@ __global__ __launch_bounds__(kBlock) void k_map(u64* __restrict__ keys,
     for (int p = 0; p < 2 * W; ++p) k0[p] = k[p];
     const double c0 = c;
     bool killed = prog.trunc_before && drop_term<W>(k, c, t);
+    if (!killed) updates += prog.idle[0];
     for (u32 j = 0; j < prog.n_ops && !killed; ++j) {
       const MapOp op = prog.ops[j];
       ++updates;
       map_apply<W>(k, c, op, prog);
       if ((op.code_flags >> 8) & 1u) killed = drop_term<W>(k, c, t);
+      if (!killed) updates += prog.idle[j + 1];
     }
     if (killed) {
       coef[i] = dead_value();

@@ -39,6 +39,8 @@ enum SmallKind : u32 { SK_ROT = 0, SK_MAP = 1, SK_SKIP = 2 };
 struct SmallStep {
   u32 kind;
   u32 arg;      // SK_MAP: index into maps, SK_SKIP: rotations in the run
+  u32 arg2;     // SK_SKIP: map ops in the run
+  u32 pad;
   RotSpec rot;  // SK_ROT
 };
 constexpr u32 kSmallStage = 32;  // step records staged in LDS at a time
@@ -352,6 +354,7 @@ __device__ __forceinline__ void small_map(const u64* P, const double* C, u64* Q,
   const u32 T = blockDim.x, tid = threadIdx.x;
   const u32 nops = small_uni(prog.n_ops);
   const bool tb = small_uni(prog.trunc_before) != 0u;
+  const bool idle = small_uni(prog.n_idle) != 0u;
   u32 out = 0;
   for (u32 base = 0; base < n; base += T) {
     const u32 i = base + tid;
@@ -362,6 +365,7 @@ __device__ __forceinline__ void small_map(const u64* P, const double* C, u64* Q,
       sload<W, CAP>(P, i, k);
       c = C[i];
       bool killed = tb && drop_term<W>(k, c, t);
+      if (idle && !killed) ty.updates += small_uni((u32)prog.idle[0]);
       for (u32 j = 0; j < nops; ++j) {
         MapOp op;
         op.code_flags = small_uni(prog.ops[j].code_flags);
@@ -371,6 +375,7 @@ __device__ __forceinline__ void small_map(const u64* P, const double* C, u64* Q,
           ++ty.updates;
           map_apply<W>(k, c, op, prog);
           if ((op.code_flags >> 8) & 1u) killed = drop_term<W>(k, c, t);
+          if (idle && !killed) ty.updates += small_uni((u32)prog.idle[j + 1]);
         }
       }
       keep = !killed;
@@ -449,9 +454,9 @@ __global__ __launch_bounds__(kSmallMaxThreads) void k_small(SmallArgs a) {
     const SmallStep& step = lsteps[s % kSmallStage];
     const u32 kind = small_uni(step.kind);
     if (kind == SK_SKIP) {
-      const u32 nskip = small_uni(step.arg);
-      uni_updates += (u64)nskip * n;
-      gates += nskip;
+      const u32 nskip = small_uni(step.arg), nskipm = small_uni(step.arg2);
+      uni_updates += (u64)(nskip + nskipm) * n;
+      gates += nskip + nskipm;
       rots += nskip;
       continue;
     }
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(kSmallMaxThreads) void k_small(SmallArgs a) {
       for (u32 j = tid; j < sizeof(MapProgram) / 4; j += T) dst[j] = src[j];
       __syncthreads();
       small_map<W, CAP>(lk[cur], lc[cur], lk[cur ^ 1], lc[cur ^ 1], wcnt, par, n, lprog, a.t, ty);
-      gates += small_uni(lprog.n_ops);
+      gates += small_uni(lprog.n_ops) + small_uni(lprog.n_idle);
       cur ^= 1u;
     }
     peak = n > peak ? n : peak;

@@ -278,6 +278,29 @@ def test_rotation_outside_support_is_dropped():
     assert c["scanned"] == 3 and c["wl_total"] == 2
 
 
+def test_map_ops_on_identity_qubits_are_dropped():
+    """A Clifford or noise op on qubits where every term holds the identity
+    changes nothing: it is dropped (no map pass) but still counts as a gate
+    and books its term updates, also between the truncation points of a map
+    program that does run."""
+    from ppvm_amd import PauliSum
+    g = PauliSum(["ZIIII", "XIIII"], 5, min_abs_coeff=1e-9)
+    g.truncate()  # the sum as a truncation leaves it
+    b = g._h.counters()
+    g.h(3)
+    g.cnot(2, 4)
+    g.pauli_error(3, p=[0.01, 0.02, 0.03])
+    c = g._h.counters()
+    assert c["gates"] - b["gates"] == 3 and c["term_updates"] - b["term_updates"] == 6
+    assert c["map_launches"] == b["map_launches"]
+    g.h(0)          # acts on both terms
+    g.s(2)          # dropped, after a truncation point of the same program
+    g.cnot(0, 1)    # acts
+    c = g._h.counters()
+    assert c["gates"] - b["gates"] == 6 and c["term_updates"] - b["term_updates"] == 12
+    assert dict(g.terms) == {"XXIII": 1.0, "ZIIII": 1.0}
+
+
 def test_determinism():
     w = WL.truncate_steps(WL.lattice_tfim(), 4)
     a, b = gpu_for(w), gpu_for(w)
