@@ -12,7 +12,8 @@ the shipped grid (same term sets; overlaps within 1e-10).
 
 Measured on one MI355X: the calibrated bound sends every new pair to the
 propagator (no validation point stays within 1e-3 under any std bound), so at
-|err| <= 1e-3 the query phase gains only what the known pairs save (1.05x).
+|err| <= 1e-3 the query phase gains only what the known pairs save (0.8-1.1x
+of the plain sweep across runs: the same propagations, timed apart).
 At the fixed std bound of 1e-2 no point falls back: the query is inference
 alone, 92% of the points within 1e-3 and all within 6.2e-3.
 """
@@ -51,9 +52,12 @@ def test_surrogate_sweep_against_live_plain_sweep():
     # overlap sum may associate differently between engine versions)
     assert rep.grid_max_diff <= 1e-10
     assert rep.n_accepted + rep.n_fallback == rep.n_new
-    # the calibrated bound keeps the bar on the query points and is not slower than the plain sweep
+    # the calibrated bound keeps the bar on the query points; with every new
+    # pair on the propagator the query phase and the plain sweep do the same
+    # propagations, so their ratio is timing noise around 1 (0.8-1.1 measured):
+    # only a query phase that repeats work would fall below the bound here
     assert rep.frac_within_bar >= 0.99
-    assert rep.speedup >= 0.9
+    assert rep.speedup >= 0.5
     # what the network gains at the fixed std bound of 1e-2 (fallbacks charged at this run's
     # plain-sweep times to the same step)
     row = next(r for r in rep.tradeoff if r["std_bound"] == 1e-2)
