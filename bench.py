@@ -772,6 +772,39 @@ def run_calibrate(args) -> dict:
             "bytes_read": 4 * CAL_ELEMENTS, "bytes_written": 4 * CAL_ELEMENTS, "dispatches": 3}
 
 
+def run_gather_ceiling(args) -> dict:
+    """What a bare gather of the rotation gather's access pattern achieves on
+    this GPU: 52.3M arena entries (32-byte keys, 8-byte coefficients: the
+    lattice workload's last arena at W = 2), 8.1% of them selected uniformly at
+    random (the anticommuting share of its last gates), their keys and
+    coefficients read and written densely (torch index_select, one kernel per
+    array).  Reported as algorithmic bytes (selected bytes read + written) per
+    second, the same measure the rotation gather's roofline uses."""
+    import torch
+    n, sel_p, reps = 52_307_346, 0.081, 10
+    g = torch.Generator(device="cuda")
+    g.manual_seed(12345)
+    keys = torch.randint(0, 2**62, (n, 4), device="cuda", dtype=torch.int64, generator=g)
+    coef = torch.rand(n, device="cuda", dtype=torch.float64, generator=g)
+    idx = torch.nonzero(torch.rand(n, device="cuda", generator=g) < sel_p).squeeze(1)
+    m = int(idx.numel())
+    for _ in range(2):
+        ok, oc = keys.index_select(0, idx), coef.index_select(0, idx)
+    torch.cuda.synchronize()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        ok, oc = keys.index_select(0, idx), coef.index_select(0, idx)
+    t1.record()
+    torch.cuda.synchronize()
+    sec = t0.elapsed_time(t1) / 1e3 / reps
+    assert torch.equal(ok[-1], keys[idx[-1]]) and float(oc[0]) == float(coef[idx[0]])
+    alg = m * (32 + 8) * 2 + m * 8  # selected keys and coefficients read and written, indices read
+    return {"metric": "bare gather of the rotation gather's access pattern", "entries": n, "selected": m,
+            "us_per_gather": sec * 1e6, "alg_bytes": alg, "alg_GBs": alg / sec / 1e9,
+            "frac_of_8TBs": alg / sec / 8e12}
+
+
 def _short_kernel(name: str) -> str:
     import re
     m = re.search(r"\b(k_[A-Za-z0-9_]+)", name)
@@ -900,7 +933,7 @@ def print_table(res: dict) -> None:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--mode", choices=["lattice", "batched", "surrogate", "parity", "lossy", "example", "configs",
-                                       "calibrate", "traffic", "kstats"], default="lattice")
+                                       "calibrate", "traffic", "kstats", "gather-ceiling"], default="lattice")
     ap.add_argument("--dir", help="kstats: rocprofv3 output dir of a --kernel-trace --stats run")
     ap.add_argument("--table", action="store_true", help="traffic / kstats: print a text table, not JSON")
     ap.add_argument("--cal-fetch", help="traffic: rocprofv3 output dir of the calibration FETCH_SIZE run")
@@ -927,7 +960,7 @@ def main() -> int:
     res = {"lattice": run, "batched": run_batched, "surrogate": run_surrogate,
            "parity": run_parity, "lossy": run_lossy, "example": run_example,
            "configs": run_configs, "calibrate": run_calibrate, "traffic": run_traffic,
-           "kstats": run_kstats}[args.mode](args)
+           "kstats": run_kstats, "gather-ceiling": run_gather_ceiling}[args.mode](args)
     if args.table and args.mode in ("traffic", "kstats"):
         print_table(res)
         return 0
